@@ -1,0 +1,142 @@
+/*
+ * ldpc_api.h -- C ABI of the MI355X QC-LDPC decoder library (libnrldpc.so).
+ *
+ * Plain C: opaque context, status codes, caller-owned buffers, plain pointers and sizes.
+ * Device pointers are HIP device memory; `stream` is a hipStream_t passed as void*
+ * (NULL = the default stream).  Every *_async call only enqueues work on `stream`.
+ * A context may be used from several host threads as long as calls on it are not
+ * concurrent with ldpc_ctx_destroy / ldpc_set_launch / ldpc_noms_set_weights.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * reference checkout).
+ */
+#ifndef NRLDPC_LDPC_API_H
+#define NRLDPC_LDPC_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------------- status codes */
+#define LDPC_OK 0
+#define LDPC_EINVAL (-1)       /* bad argument: unsupported (kind, Z, rows), sizes, NULLs */
+#define LDPC_ENOMEM (-2)       /* allocation failed or batch larger than the context allows */
+#define LDPC_EHIP (-3)         /* HIP runtime error; see ldpc_last_error() */
+#define LDPC_ENODEV (-4)       /* no usable GPU */
+#define LDPC_EUNSUPPORTED (-5) /* valid request that this build does not implement */
+
+/* ----------------------------------------------------------------- codes */
+#define LDPC_CODE_BG1 1        /* 5G NR base graph 1: 46 x 68, K = 22 Z */
+#define LDPC_CODE_BG2 2        /* 5G NR base graph 2: 42 x 52, K = 10 Z */
+#define LDPC_CODE_QC12X24 3    /* 12 x 24 QC example code (802.11n rate-1/2 prototype), K = 12 Z */
+
+#define LDPC_FLAG_EARLY_EXIT 1u /* stop a codeword once its hard decision has zero syndrome */
+
+typedef struct ldpc_ctx ldpc_ctx;
+
+typedef struct {
+    int kind, Z, ils;  /* code kind, lifting size, TS 38.212 lifting-set index */
+    int rows, cols;    /* base-graph rows / columns in use */
+    int kb;            /* information block columns */
+    int punct;         /* punctured leading block columns (2 for NR, 0 for the 12x24 code) */
+    int n_vn;          /* decoder columns = cols * Z (LLR entries per codeword) */
+    int K;             /* information bits = kb * Z (incl. punctured) */
+    int N;             /* transmitted bits = (cols - punct) * Z */
+    int hd_bytes;      /* packed hard-decision bytes per codeword = ceil(n_vn / 8) */
+    int nnz;           /* block edges in use */
+    int cw_per_block;  /* codewords per workgroup used by the decoder launch */
+} ldpc_code_info;
+
+const char* ldpc_strerror(int status);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* ldpc_last_error(void);
+int ldpc_device_count(int* count);
+
+/* Create a decoder context for one code on one device.
+ * Replaces the one-code-per-process setup of the reference:
+ *   setup_ldpc                         src/nr_ldpc.cpp:31-230
+ *   put_h_compact_and_params_in_constant_memory  src/setup_exp.cu:286-312
+ *   allocate_constant_memory / put_params_in_constant_mem  src/ldpc_kernels.cu:29-57
+ * m_rows: kept base-graph parity rows (rate matching by row truncation); <= 0 keeps all. */
+int ldpc_ctx_create(ldpc_ctx** out, int kind, int Z, int m_rows, int device);
+int ldpc_ctx_destroy(ldpc_ctx* ctx);
+int ldpc_ctx_info(const ldpc_ctx* ctx, ldpc_code_info* info);
+
+/* Decode n_cw codewords with flooding scaled min-sum (alpha = 0.75), n_iter iterations.
+ * Replaces perform_gpu_ldpc_decoding<T>  src/gpu_decoder_top.cu:16-52 and the kernels
+ * cnp_kernel_1st_iter / cnp_kernel / vnp_kernel_normal / vnp_kernel_last_iter /
+ * pack_hard_decision  src/ldpc_kernels.cu:72-500.
+ *   llr        device, [n_cw][n_vn] float32 channel LLRs (punctured columns = 0)
+ *   hd_packed  device, [n_cw][hd_bytes]; bit k of byte b = hard decision of column 8b+k
+ *              (1 when APP <= 0), the bit order of unpack_char src/host_helpers.cpp:60-64
+ *   app_out    device, [n_cw][n_vn] posterior LLRs, or NULL
+ *   iters_out  device, [n_cw] iterations run, or NULL
+ *   flags      LDPC_FLAG_EARLY_EXIT or 0 */
+int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
+                      unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+
+/* Codewords per workgroup of the decode launch (0 restores the context's default).
+ * No reference counterpart: the reference fixes n_cw_per_mcw at build time
+ * (setup_gpu_grid src/setup_exp.cu:131-166). */
+int ldpc_set_launch(ldpc_ctx* ctx, int cw_per_block);
+
+/* Syndrome weight (unsatisfied checks) of packed hard decisions, one int per codeword.
+ * No reference counterpart (early termination is a TODO in the reference, README.md:193). */
+int ldpc_syndrome_async(ldpc_ctx* ctx, void* stream, const uint8_t* hd_packed, int n_cw,
+                        int* weight_out);
+
+/* ----------------------------------------------------------------- host-side code utilities */
+
+/* Raw base-matrix table (rows x cols ints, -1 = zero block) for (kind, ils).
+ * Replaces the h_base_{1,2}_i{0..7} globals  src/h_matrices/h_bg_*.h (with the table
+ * defects of BG1 i_LS 2..7 repaired). */
+int ldpc_base_matrix(int kind, int ils, int* out, int out_len);
+
+/* TS 38.212 parameter derivation for payload A and nominal rate.
+ * Replaces setup_ldpc  src/nr_ldpc.cpp:31-230.  out[10] = {bg, Z, K, N, H_rows, H_cols,
+ * ils, N_before_puncture, n_nz_in_row, n_nz_in_col}. */
+int ldpc_setup_params(int A, float rate, int* out);
+
+/* Systematic encoder on the host: info [n_cw][K] bits (one byte each) -> codeword
+ * [n_cw][n_vn] bits (one byte each, punctured columns included) with H c = 0.
+ * Replaces ldpc_encoder_orig  src/nr_ldpc.cpp:655-913 (which, for BG2 at full rate, leaves
+ * the last 20 parity rows unencoded; this encoder solves every kept row). */
+int ldpc_encode_host(const ldpc_ctx* ctx, const uint8_t* info, int n_cw, uint8_t* cw);
+
+/* Generator shift lists regenerated from H for (kind, Z): fills no_shift[rows*kb] and
+ * pointer_shift[rows*kb] (as the OAI tables lay them out) and returns the number of shift
+ * values written to gen_shift (at most gen_len), or a negative status.
+ * Replaces choose_generator_matrix  src/nr_ldpc.cpp:234-652 and the tables of
+ * src/Gen_shift_value.h. */
+int ldpc_generator_shifts(int kind, int Z, short* no_shift, short* pointer_shift, short* gen_shift,
+                          int gen_len);
+
+/* ----------------------------------------------------------------- on-device channel */
+
+/* Noise standard deviation for Eb/N0 (dB) and nominal rate.
+ * Replaces compute_noise_stddev  src/encoding_tools.cpp:27-38. */
+float ldpc_noise_stddev(float ebn0_db, float rate);
+
+/* Random codewords and channel LLRs generated on the device with Philox4x32-10:
+ * info bits from (seed, stream_base + i), systematic encoding, BPSK (0 -> +1),
+ * y = x + sigma * n, LLR = (2 y) / (sigma^2), punctured columns 0.
+ * sigma: one value per codeword (device array) or, if sigma_dev is NULL, sigma_all.
+ *   cw_bits   device [n_cw][n_vn] bytes (may be NULL)   llr  device [n_cw][n_vn] floats
+ * Replaces the host chain generate_new_codewords -> transmit -> calculate_llr
+ *   src/setup_exp.cu:168-196, src/encoding_tools.cpp:13-99. */
+int ldpc_channel_async(ldpc_ctx* ctx, void* stream, uint64_t seed, uint64_t stream_base, int n_cw,
+                       const float* sigma_dev, float sigma_all, uint8_t* cw_bits, float* llr);
+
+/* Bit / frame errors of decoded codewords against cw_bits over the first K columns
+ * (info bits incl. punctured ones), accumulated into counts[2] = {bit_errors, frame_errors}
+ * (device int64).  Replaces cuda_error_check  src/encoding_tools.cpp:112-155. */
+int ldpc_count_errors_async(ldpc_ctx* ctx, void* stream, const uint8_t* hd_packed,
+                            const uint8_t* cw_bits, int n_cw, long long* counts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,38 @@
+// decoder.h -- internal launch interface of the fused decoder kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace nrldpc {
+
+constexpr int kMaxBlock = 512;          // threads per workgroup (2 waves per SIMD)
+constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
+
+// Per base-graph edge, in row-major edge order.  Byte offsets are relative to the lane's
+// read-copy / accumulator base (see decoder.hip).
+struct alignas(16) EdgeParam {
+    int off_rd;   // (nd_col * 2Z + V) * 4
+    int off_acc;  // (nd_col * Z + V) * 4
+    int thr;      // Z - V: lanes with r >= thr wrap around the circulant
+    int vm;       // V mod Z
+};
+
+struct DecodeArgs {
+    const float* llr;      // [n_cw][n_vn]
+    uint8_t* hd;           // [n_cw][hd_bytes]
+    float* app_out;        // [n_cw][n_vn] or null
+    int* iters_out;        // [n_cw] or null
+    const EdgeParam* etab; // [nnz_full]
+    const float2* weights; // learned: [(n_iter+1)][nnz] (alpha, beta); row 0 all zero
+    int n_cw, Z, m_rows, n_vn, hd_bytes, n_iter, cpb, early_exit;
+    int lds_cw_bytes;
+    int nnz;
+};
+
+size_t decoder_lds_bytes_per_cw(int kind, int Z);
+int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
+
+}  // namespace nrldpc
