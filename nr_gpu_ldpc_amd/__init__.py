@@ -1,0 +1,175 @@
+"""MI355X-native QC-LDPC decoder: Python bindings of the C ABI in include/ldpc_api.h.
+
+The library ``nr_gpu_ldpc_amd/lib/libnrldpc.so`` is built by ``nr_gpu_ldpc_amd/build.py``
+(hipcc, gfx950).  Importing this package fails loudly if it has not been built: there is no
+CPU or PyTorch fallback for any decode path.
+
+Device buffers are passed as integer device addresses (for example ``tensor.data_ptr()`` of a
+torch tensor on ``cuda:0``) and streams as integer hipStream_t handles (0 = default stream).
+"""
+import ctypes
+import os
+
+__all__ = ["lib", "Code", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "setup_params",
+           "base_matrix", "noise_stddev", "generator_shifts", "device_count"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnrldpc.so")
+
+BG1, BG2, QC12X24 = 1, 2, 3
+EARLY_EXIT = 1
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build the HIP library first (python -m nr_gpu_ldpc_amd.build)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+
+
+class CodeInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in
+                ("kind", "Z", "ils", "rows", "cols", "kb", "punct", "n_vn", "K", "N", "hd_bytes",
+                 "nnz", "cw_per_block")]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+_P = ctypes.c_void_p
+_sig("ldpc_strerror", ctypes.c_char_p, ctypes.c_int)
+_sig("ldpc_last_error", ctypes.c_char_p)
+_sig("ldpc_device_count", ctypes.c_int, _c_int_p)
+_sig("ldpc_ctx_create", ctypes.c_int, ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+     ctypes.c_int)
+_sig("ldpc_ctx_destroy", ctypes.c_int, _P)
+_sig("ldpc_ctx_info", ctypes.c_int, _P, ctypes.POINTER(CodeInfo))
+_sig("ldpc_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
+_sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_base_matrix", ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_int)
+_sig("ldpc_setup_params", ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_int_p)
+_sig("ldpc_encode_host", ctypes.c_int, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_generator_shifts", ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int)
+_sig("ldpc_noise_stddev", ctypes.c_float, ctypes.c_float, ctypes.c_float)
+_sig("ldpc_channel_async", ctypes.c_int, _P, _P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _P,
+     ctypes.c_float, _P, _P)
+_sig("ldpc_count_errors_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, _P)
+
+
+class LdpcError(RuntimeError):
+    def __init__(self, status, what):
+        msg = lib.ldpc_strerror(status).decode()
+        detail = lib.ldpc_last_error().decode()
+        super().__init__(f"{what}: {msg}" + (f" ({detail})" if detail else ""))
+        self.status = status
+
+
+def _check(status, what):
+    if status != 0:
+        raise LdpcError(status, what)
+
+
+def device_count():
+    n = ctypes.c_int()
+    _check(lib.ldpc_device_count(ctypes.byref(n)), "ldpc_device_count")
+    return n.value
+
+
+def setup_params(A, rate):
+    """setup_ldpc (src/nr_ldpc.cpp:31-230): dict of the derived code parameters."""
+    out = (ctypes.c_int * 10)()
+    _check(lib.ldpc_setup_params(int(A), float(rate), out), "ldpc_setup_params")
+    keys = ("bg", "Z", "K", "N", "H_rows", "H_cols", "ils", "N_before_puncture", "n_nz_in_row",
+            "n_nz_in_col")
+    return dict(zip(keys, list(out)))
+
+
+def base_matrix(kind, ils):
+    import numpy as np
+    rows, cols = {BG1: (46, 68), BG2: (42, 52), QC12X24: (12, 24)}[kind]
+    out = np.empty((rows, cols), dtype=np.int32)
+    _check(lib.ldpc_base_matrix(kind, ils, out.ctypes.data_as(_c_int_p), out.size), "ldpc_base_matrix")
+    return out
+
+
+def noise_stddev(ebn0_db, rate):
+    return lib.ldpc_noise_stddev(float(ebn0_db), float(rate))
+
+
+def generator_shifts(kind, Z):
+    """Regenerated OAI-layout generator tables: (no_shift, pointer_shift, gen_shift) int16 arrays."""
+    import numpy as np
+    n = lib.ldpc_generator_shifts(kind, Z, None, None, None, 0)
+    if n < 0:
+        _check(n, "ldpc_generator_shifts")
+    kb = 22 if kind == BG1 else 10
+    rows = 46 if kind == BG1 else 42
+    no = np.zeros(rows * kb, dtype=np.int16)
+    ptr = np.zeros(rows * kb, dtype=np.int16)
+    gen = np.zeros(n, dtype=np.int16)
+    lib.ldpc_generator_shifts(kind, Z, no.ctypes.data, ptr.ctypes.data, gen.ctypes.data, n)
+    return no, ptr, gen
+
+
+class Code:
+    """One decoder context (code kind, lifting size Z, kept parity rows) on one device."""
+
+    def __init__(self, kind, Z, m_rows=0, device=0):
+        h = _P()
+        _check(lib.ldpc_ctx_create(ctypes.byref(h), kind, Z, m_rows, device), "ldpc_ctx_create")
+        self._h = h
+        info = CodeInfo()
+        _check(lib.ldpc_ctx_info(h, ctypes.byref(info)), "ldpc_ctx_info")
+        self.info = {f: getattr(info, f) for f, _ in CodeInfo._fields_}
+        for k, v in self.info.items():
+            setattr(self, k, v)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ldpc_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_launch(self, cw_per_block):
+        _check(lib.ldpc_set_launch(self._h, int(cw_per_block)), "ldpc_set_launch")
+        self.cw_per_block = cw_per_block or self.info["cw_per_block"]
+
+    def decode(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        _check(lib.ldpc_decode_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,
+                                     app_ptr or None, iters_ptr or None), "ldpc_decode_async")
+
+    def syndrome(self, hd_ptr, n_cw, weight_ptr, stream=0):
+        _check(lib.ldpc_syndrome_async(self._h, stream, hd_ptr, n_cw, weight_ptr), "ldpc_syndrome_async")
+
+    def channel(self, seed, stream_base, n_cw, llr_ptr, sigma=None, sigma_ptr=0, cw_bits_ptr=0, stream=0):
+        _check(lib.ldpc_channel_async(self._h, stream, seed, stream_base, n_cw, sigma_ptr or None,
+                                      float(sigma or 0.0), cw_bits_ptr or None, llr_ptr),
+               "ldpc_channel_async")
+
+    def count_errors(self, hd_ptr, cw_bits_ptr, n_cw, counts_ptr, stream=0):
+        _check(lib.ldpc_count_errors_async(self._h, stream, hd_ptr, cw_bits_ptr, n_cw, counts_ptr),
+               "ldpc_count_errors_async")
+
+    def encode_host(self, info_bits):
+        """info_bits: uint8 array [n_cw, K] -> codeword bits uint8 [n_cw, n_vn] (host)."""
+        import numpy as np
+        info_bits = np.ascontiguousarray(info_bits, dtype=np.uint8).reshape(-1, self.K)
+        out = np.empty((info_bits.shape[0], self.n_vn), dtype=np.uint8)
+        _check(lib.ldpc_encode_host(self._h, info_bits.ctypes.data, info_bits.shape[0], out.ctypes.data),
+               "ldpc_encode_host")
+        return out

@@ -1,0 +1,44 @@
+// channel.h -- launch interface of the channel, syndrome and error-count kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace nrldpc {
+
+struct ChannelArgs {
+    uint64_t seed, stream_base;
+    int Z, n_vn, K, punct;
+    int n_steps;
+    const int* step_target;  // [n_steps]
+    const int* step_start;   // [n_steps + 1]
+    const int* term_col;
+    const int* term_shift;
+    const float* sigma_dev;  // [n_cw] or null
+    float sigma_all;
+    uint8_t* cw_bits;        // [n_cw][n_vn] or null
+    float* llr;              // [n_cw][n_vn]
+};
+
+struct SyndromeArgs {
+    const uint8_t* hd;
+    int hd_bytes, Z, rows;
+    const int* row_start;  // [rows + 1]
+    const int* edge_col;   // per edge, row-major
+    const int* edge_vm;
+    int* weight;
+};
+
+struct ErrorArgs {
+    const uint8_t* hd;
+    const uint8_t* cw_bits;
+    int hd_bytes, n_vn, K;
+    long long* counts;
+};
+
+int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s);
+int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s);
+int count_errors_launch(const ErrorArgs& a, int n_cw, hipStream_t s);
+
+}  // namespace nrldpc

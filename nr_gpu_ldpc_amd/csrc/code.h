@@ -54,6 +54,9 @@ template <int KIND> struct Structure {
         int d1_edge[COLS];     // the single edge of a degree-1 column, else -1
         int d1_row[COLS];      // its block row, else -1
         int epoch_start[ROWS]; // 1 when row i must wait for all previous rows' VN updates
+        int w_word[ROWS];      // register holding row i's sign bits and arg-min index
+        int w_off[ROWS];       // bit offset of row i inside that register (0 or 16)
+        int n_w;               // registers used for all rows
         int max_deg;
         int n_nd, n_d1;
     };
@@ -85,6 +88,21 @@ template <int KIND> struct Structure {
             const int d = t.row_start[i + 1] - t.row_start[i];
             if (d > t.max_deg) t.max_deg = d;
         }
+        // Sign bits (one per edge) and the 5-bit arg-min index of a row of degree <= 11 fit in
+        // 16 bits (signs at bits 0..10, index at 11..15): two such rows share one register.
+        // Wider rows use a full register (signs at 0..d-1, index at 24..28).
+        t.n_w = 0;
+        int half_open = -1;
+        for (int i = 0; i < ROWS; i++) {
+            const int d = t.row_start[i + 1] - t.row_start[i];
+            if (d <= 11) {
+                if (half_open >= 0) { t.w_word[i] = half_open; t.w_off[i] = 16; half_open = -1; }
+                else { t.w_word[i] = t.n_w++; t.w_off[i] = 0; half_open = t.w_word[i]; }
+            } else {
+                t.w_word[i] = t.n_w++;
+                t.w_off[i] = 0;
+            }
+        }
         // Rows that share no LDS-resident column with the rows since the last epoch start can
         // update variable nodes without waiting: per-column update order stays ascending.
         bool used[COLS] = {};
@@ -105,6 +123,8 @@ template <int KIND> struct Structure {
     static constexpr int NND = T.n_nd;
     static constexpr int ND1 = T.n_d1;
     static constexpr int MAXDEG = T.max_deg;
+    static constexpr int NW = T.n_w;
+    static constexpr bool half(int i) { return deg(i) <= 11; }
 };
 
 // Every LDS-resident column lies in the information + core part, which parity-row truncation
@@ -119,6 +139,8 @@ static_assert(nd_columns_in_core<kBG1>() && nd_columns_in_core<kBG2>(), "unexpec
 static_assert(Structure<kBG1>::NND == 26 && Structure<kBG1>::ND1 == 42, "BG1 structure");
 static_assert(Structure<kBG2>::NND == 14 && Structure<kBG2>::ND1 == 38, "BG2 structure");
 static_assert(Structure<kWifi>::NND == 24 && Structure<kWifi>::ND1 == 0, "12x24 structure");
+static_assert(Structure<kBG1>::NW == 25 && Structure<kBG2>::NW == 21 && Structure<kWifi>::NW == 6,
+              "packed sign words");
 
 // ------------------------------------------------------------------ run-time description
 

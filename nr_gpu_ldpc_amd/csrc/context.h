@@ -1,0 +1,43 @@
+// context.h -- the opaque ldpc_ctx of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/ldpc_api.h"
+#include "code.h"
+#include "decoder.h"
+#include "encoder.h"
+
+struct ldpc_ctx {
+    int device = 0;
+    nrldpc::CodeDesc code;
+    nrldpc::EncodeProgram prog;
+    int cpb = 1, cpb_default = 1;  // codewords per workgroup
+    // device tables
+    nrldpc::EdgeParam* d_etab = nullptr;
+    int* d_row_start = nullptr;
+    int* d_edge_col = nullptr;
+    int* d_edge_vm = nullptr;
+    int* d_step_target = nullptr;
+    int* d_step_start = nullptr;
+    int* d_term_col = nullptr;
+    int* d_term_shift = nullptr;
+    // learned offset/normalized min-sum weights: [(n_iter + 1)][nnz_full] float2, row 0 zero
+    float2* d_weights = nullptr;
+    int weights_iters = 0;
+    std::mutex mu;
+};
+
+namespace nrldpc {
+
+int nnz_full(int kind);
+int n_nd(int kind);
+int default_cpb(const CodeDesc& c);
+// Shared decode path; weights != null selects the learned decoder with [(n_iter+1)][nnz] weights.
+int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
+                uint8_t* hd, float* app_out, int* iters_out, const float2* weights);
+
+}  // namespace nrldpc

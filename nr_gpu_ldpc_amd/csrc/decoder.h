@@ -11,13 +11,14 @@ namespace nrldpc {
 constexpr int kMaxBlock = 512;          // threads per workgroup (2 waves per SIMD)
 constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
 
-// Per base-graph edge, in row-major edge order.  Byte offsets are relative to the lane's
-// read-copy / accumulator base (see decoder.hip).
-struct alignas(16) EdgeParam {
-    int off_rd;   // (nd_col * 2Z + V) * 4
-    int off_acc;  // (nd_col * Z + V) * 4
-    int thr;      // Z - V: lanes with r >= thr wrap around the circulant
-    int vm;       // V mod Z
+// Per base-graph edge, in row-major edge order (two dwords, read through the scalar cache).
+// For an edge into an LDS-resident column (nd = block column j < NND):
+//   off_rd = (j * 2Z + V) * 4                 byte offset into the read copy
+//   acc    = (Z - V) << 16 | (j * Z + V) * 4  wrap threshold | byte offset into the accumulator
+// For an edge into a degree-1 column: off_rd = V (mod Z), acc = 0.
+struct alignas(8) EdgeParam {
+    uint32_t off_rd;
+    uint32_t acc;
 };
 
 struct DecodeArgs {

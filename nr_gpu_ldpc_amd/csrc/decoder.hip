@@ -33,6 +33,10 @@
 #include <type_traits>
 #include <utility>
 
+#ifndef NRLDPC_LDS_ATOMIC_ACC
+#define NRLDPC_LDS_ATOMIC_ACC 0
+#endif
+
 namespace nrldpc {
 
 template <int B, int E, class F>
@@ -60,9 +64,22 @@ __device__ __forceinline__ void lds_acc(uint32_t a, float dt) {
     *p = *p + dt;
 }
 
+// Per-edge table entries are read through the constant address space so that the compiler
+// issues scalar (s_load) reads of these wave-uniform values.
+using CEdge = const EdgeParam __attribute__((address_space(4)));
+
+template <int KIND>
+constexpr bool nd_columns_leading() {
+    using S = Structure<KIND>;
+    for (int j = 0; j < S::NND; j++)
+        if (S::T.nd_index[j] != j) return false;
+    return true;
+}
+
 template <int KIND, bool LEARNED>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     using S = Structure<KIND>;
+    static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -76,37 +93,40 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     const int cwc = live ? cw : a.n_cw - 1;
     const uint32_t Z4 = 4u * Z;
     const uint32_t base = cwl * a.lds_cw_bytes;
-    const uint32_t rRd = base + 4u * r;                  // read copy, block column 0
-    const uint32_t rAcc = base + 2u * NND * Z4 + 4u * r; // accumulator, block column 0
-    const uint32_t rAccW = rAcc - Z4;                    // same, for lanes whose (r+V) wraps
+    uint32_t rRd = base + 4u * r;                  // read copy, block column 0
+    uint32_t rAcc = base + 2u * NND * Z4 + 4u * r; // accumulator, block column 0
+    uint32_t rAccW = rAcc - Z4;                    // same, for lanes whose (r+V) wraps
+    const uint32_t rHi = ((uint32_t)r << 16) | 0xFFFFu;  // compared with EdgeParam::acc
     const uint32_t flagAddr = base + 3u * NND * Z4;
-    const EdgeParam* __restrict__ et = a.etab;
+    CEdge* et = (CEdge*)(a.etab);
     const int m_rows = a.m_rows;
 
+    constexpr int NW = S::NW;
     float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
-    uint32_t W[ROWS];          // bits 0..d-1: sign of R per edge; bits 24..28: arg-min index
+    uint32_t W[NW];            // per row: sign of R per edge + arg-min index (see Structure)
     float app1[ND1];           // APP of the degree-1 columns fed by this lane's rows
 #pragma unroll
-    for (int i = 0; i < ROWS; i++) { A1[i] = 0.f; A2[i] = 0.f; W[i] = 0u; }
+    for (int i = 0; i < ROWS; i++) { A1[i] = 0.f; A2[i] = 0.f; }
+#pragma unroll
+    for (int i = 0; i < NW; i++) W[i] = 0u;
 
     // ---- load channel LLRs
     const float* __restrict__ L = a.llr + (size_t)cwc * a.n_vn;
-    static_for<0, S::COLS>([&](auto JC) {
+#pragma unroll 2
+    for (int j = 0; j < NND; j++) {
+        const float v = L[j * Z + r];
+        lds_st(rRd + j * 2 * Z4, v);
+        lds_st(rRd + j * 2 * Z4 + Z4, v);
+        lds_st(rAcc + j * Z4, v);
+    }
+    static_for<NND, S::COLS>([&](auto JC) {
         constexpr int j = JC;
-        if constexpr (S::T.nd_index[j] >= 0) {
-            constexpr int jj = S::T.nd_index[j];
-            const float v = L[j * Z + r];
-            lds_st(rRd + jj * 2 * Z4, v);
-            lds_st(rRd + jj * 2 * Z4 + Z4, v);
-            lds_st(rAcc + jj * Z4, v);
-        } else {
-            constexpr int e = S::T.d1_edge[j];
-            constexpr int i = S::T.d1_row[j];
-            if (i < m_rows) {
-                int c = r + et[e].vm;
-                if (c >= Z) c -= Z;
-                app1[S::T.d1_slot[j]] = L[j * Z + c];
-            }
+        constexpr int e = S::T.d1_edge[j];
+        constexpr int i = S::T.d1_row[j];
+        if (i < m_rows) {
+            int c = r + (int)et[e].off_rd;
+            if (c >= Z) c -= Z;
+            app1[S::T.d1_slot[j]] = L[j * Z + c];
         }
     });
     __syncthreads();
@@ -118,26 +138,32 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     // Writes hard decisions (and APP) of every lane's codeword for which `mine` holds.
     auto emit = [&](bool mine, int iters) {
         if (mine) {
-            static_for<0, S::COLS>([&](auto JC) {
+            // Loop-variant copies: no address of this path is precomputed outside it.
+            uint32_t eb = base;
+            int er = r;
+            float* ap_out = a.app_out;
+            CEdge* ee = et;
+            asm volatile("" : "+v"(eb), "+v"(er), "+s"(ap_out), "+s"(ee));
+            if (ap_out) ap_out += (size_t)cw * a.n_vn;
+#pragma unroll 1
+            for (int j = 0; j < NND; j++) {
+                const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                g_smem[eb + j * Z + er] = x > 0.f ? 0 : 1;
+                if (ap_out) ap_out[j * Z + er] = x;
+            }
+            static_for<NND, S::COLS>([&](auto JC) {
                 constexpr int j = JC;
-                if constexpr (S::T.nd_index[j] >= 0) {
-                    constexpr int jj = S::T.nd_index[j];
-                    const float x = lds_ld(rAcc + jj * Z4);
-                    g_smem[base + j * Z + r] = x > 0.f ? 0 : 1;
-                    if (a.app_out) a.app_out[(size_t)cw * a.n_vn + j * Z + r] = x;
-                } else {
-                    constexpr int e = S::T.d1_edge[j];
-                    constexpr int i = S::T.d1_row[j];
-                    if (i < m_rows) {
-                        int c = r + et[e].vm;
-                        if (c >= Z) c -= Z;
-                        const float x = app1[S::T.d1_slot[j]];
-                        g_smem[base + j * Z + c] = x > 0.f ? 0 : 1;
-                        if (a.app_out) a.app_out[(size_t)cw * a.n_vn + j * Z + c] = x;
-                    }
+                constexpr int e = S::T.d1_edge[j];
+                constexpr int i = S::T.d1_row[j];
+                if (i < m_rows) {
+                    int c = er + (int)ee[e].off_rd;
+                    if (c >= Z) c -= Z;
+                    const float x = app1[S::T.d1_slot[j]];
+                    g_smem[eb + j * Z + c] = x > 0.f ? 0 : 1;
+                    if (ap_out) ap_out[j * Z + c] = x;
                 }
             });
-            if (r < 8) g_smem[base + a.n_vn + r] = 0;
+            if (er < 8) g_smem[eb + a.n_vn + er] = 0;
         }
         __syncthreads();
         if (mine) {
@@ -154,35 +180,46 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
 
     int t = 0;
     for (; t < a.n_iter; ++t) {
+        // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
+        // out of the iteration loop into long-lived VGPRs.
+        asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW));
         const float2* __restrict__ w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
         const float2* __restrict__ w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
         static_for<0, ROWS>([&](auto IC) {
             constexpr int i = IC;
             constexpr int D = S::deg(i);
             constexpr int E0 = S::T.row_start[i];
-            if (i >= m_rows) return;
+            constexpr int WW = S::T.w_word[i];
+            constexpr int WO = S::T.w_off[i];                 // sign bits start here
+            constexpr int IO = S::half(i) ? WO + 11 : 24;     // arg-min index starts here
+            int mr = m_rows;
+            asm volatile("" : "+s"(mr));
+            if (i >= mr) return;
+            // Keep this row's table reads inside the row (bounded SGPR live ranges).
+            asm volatile("" : "+s"(et));
             float q[D], ro[D];
             float m1 = 1000.0f, m2 = 1000.0f;
             uint32_t neg = 0u;
             const float a1 = A1[i], a2 = A2[i];
-            const uint32_t wo = W[i];
-            const uint32_t idxo = wo >> 24;
-            // pass 1: Q, two-min, signs
+            const uint32_t wo = W[WW];
+            uint32_t idxo = (wo >> IO) & 31u;
+            asm volatile("" : "+v"(idxo));
+            // pass 1: Q = APP_old - R_old, two-min, signs
             static_for<0, D>([&](auto KC) {
                 constexpr int k = KC;
                 constexpr int e = E0 + k;
                 constexpr int j = S::T.ecol[e];
                 float x;
-                if constexpr (S::T.nd_index[j] >= 0) x = lds_ld(rRd + et[e].off_rd);
+                if constexpr (j < NND) x = lds_ld(rRd + et[e].off_rd);
                 else x = app1[S::T.d1_slot[j]];
                 const float mo = (idxo == (uint32_t)k) ? a2 : a1;
                 float rv;
                 if constexpr (LEARNED) {
                     const float2 w = w_old[e];
                     const float mag = w.x * fmaxf(mo - w.y, 0.0f);
-                    rv = as_f(as_u(mag) ^ ((wo << (31 - k)) & 0x80000000u));
+                    rv = as_f(as_u(mag) ^ ((wo << (31 - WO - k)) & 0x80000000u));
                 } else {
-                    rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (31 - k)) & 0x80000000u));
+                    rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (31 - WO - k)) & 0x80000000u));
                 }
                 const float qv = x - rv;
                 q[k] = qv;
@@ -193,49 +230,94 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                 neg |= (qv < 0.0f ? 1u : 0u) << k;
             });
             const uint32_t sgn = __popc(neg) & 1u;
+            const uint32_t sbit = sgn << 31;
             float n1, n2;
             if constexpr (LEARNED) { n1 = m1; n2 = m2; }
             else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
-            if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
-            // pass 2: R_new, dt, VN update
+            const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
+            // pass 2: R_new = (product of the other signs) * scaled min, dt = R_new - R_old
             uint32_t idxn = 0u;
+            float dtv[D];
+            static_for<0, D>([&](auto KC) {
+                constexpr int k = KC;
+                constexpr int e = E0 + k;
+                const bool eq = fabsf(q[k]) == m1;
+                idxn = eq ? (uint32_t)k : idxn;
+                float ms;
+                if constexpr (LEARNED) {
+                    const float2 w = w_new[e];
+                    const float mag = w.x * fmaxf((eq ? n2 : n1) - w.y, 0.0f);
+                    ms = as_f(as_u(mag) ^ sbit);
+                } else {
+                    ms = eq ? n2s : n1s;
+                }
+                const float rn = (q[k] < 0.0f) ? -ms : ms;
+                dtv[k] = rn - ro[k];
+            });
+            A1[i] = n1;
+            A2[i] = n2;
+            constexpr uint32_t dmask = (1u << D) - 1u;
+            const uint32_t fresh_w = ((sgn ? ~neg : neg) & dmask) | (idxn << (IO - WO));
+            if constexpr (!S::half(i)) W[WW] = fresh_w;
+            else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
+            else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
+            if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
+            // VN update, ascending block row per column
+#if NRLDPC_LDS_ATOMIC_ACC
             static_for<0, D>([&](auto KC) {
                 constexpr int k = KC;
                 constexpr int e = E0 + k;
                 constexpr int j = S::T.ecol[e];
-                const bool eq = fabsf(q[k]) == m1;
-                idxn = eq ? (uint32_t)k : idxn;
-                float mag = eq ? n2 : n1;
-                if constexpr (LEARNED) {
-                    const float2 w = w_new[e];
-                    mag = w.x * fmaxf(mag - w.y, 0.0f);
-                }
-                const uint32_t s = (((neg >> k) ^ sgn) & 1u) << 31;
-                const float rn = as_f(as_u(mag) ^ s);
-                const float dt = rn - ro[k];
-                if constexpr (S::T.nd_index[j] >= 0) {
-                    const EdgeParam p = et[e];
-                    const uint32_t addr = (r >= p.thr ? rAccW : rAcc) + p.off_acc;
-                    lds_acc(addr, dt);
+                if constexpr (j < NND) {
+                    const uint32_t acc = et[e].acc;
+                    const uint32_t addr = (rHi >= acc ? rAccW : rAcc) + (acc & 0xFFFFu);
+                    __hip_atomic_fetch_add(reinterpret_cast<float*>(g_smem + addr), dtv[k],
+                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
                     float& ap = app1[S::T.d1_slot[j]];
-                    ap = ap + dt;
+                    ap = ap + dtv[k];
                 }
             });
-            A1[i] = n1;
-            A2[i] = n2;
-            constexpr uint32_t dmask = (D >= 32) ? 0xffffffffu : ((1u << D) - 1u);
-            W[i] = ((sgn ? ~neg : neg) & dmask) | (idxn << 24);
+#else
+            // read-modify-write in chunks of 8 edges: loads first, then stores
+            static_for<0, (D + 7) / 8>([&](auto CC) {
+                constexpr int C0 = CC * 8;
+                constexpr int CN = (D - C0) < 8 ? (D - C0) : 8;
+                uint32_t addr[CN];
+                float cur[CN];
+                static_for<0, CN>([&](auto KC) {
+                    constexpr int k = C0 + KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    if constexpr (j < NND) {
+                        const uint32_t acc = et[e].acc;
+                        addr[KC] = (rHi >= acc ? rAccW : rAcc) + (acc & 0xFFFFu);
+                        cur[KC] = lds_ld(addr[KC]);
+                    }
+                });
+                static_for<0, CN>([&](auto KC) {
+                    constexpr int k = C0 + KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    if constexpr (j < NND) {
+                        lds_st(addr[KC], cur[KC] + dtv[k]);
+                    } else {
+                        float& ap = app1[S::T.d1_slot[j]];
+                        ap = ap + dtv[k];
+                    }
+                });
+            });
+#endif
         });
         __syncthreads();
         if (t + 1 == a.n_iter) break;
         // APP_new becomes the read copy of the next iteration.
-        static_for<0, NND>([&](auto JJ) {
-            constexpr int jj = JJ;
-            const float v = lds_ld(rAcc + jj * Z4);
-            lds_st(rRd + jj * 2 * Z4, v);
-            lds_st(rRd + jj * 2 * Z4 + Z4, v);
-        });
+#pragma unroll 2
+        for (int j = 0; j < NND; j++) {
+            const float v = lds_ld(rAcc + j * Z4);
+            lds_st(rRd + j * 2 * Z4, v);
+            lds_st(rRd + j * 2 * Z4 + Z4, v);
+        }
         if (early && r == 0) lds_stu(flagAddr, 0u);
         __syncthreads();
         if (early) {
@@ -245,13 +327,16 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                 constexpr int i = IC;
                 constexpr int D = S::deg(i);
                 constexpr int E0 = S::T.row_start[i];
-                if (i >= m_rows) return;
+                int mr = m_rows;
+                asm volatile("" : "+s"(mr));
+                if (i >= mr) return;
+                asm volatile("" : "+s"(et));
                 uint32_t p = 0u;
                 static_for<0, D>([&](auto KC) {
                     constexpr int e = E0 + KC;
                     constexpr int j = S::T.ecol[e];
                     float x;
-                    if constexpr (S::T.nd_index[j] >= 0) x = lds_ld(rRd + et[e].off_rd);
+                    if constexpr (j < NND) x = lds_ld(rRd + et[e].off_rd);
                     else x = app1[S::T.d1_slot[j]];
                     p ^= x > 0.f ? 0u : 1u;
                 });
@@ -261,11 +346,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             __syncthreads();
             const bool conv = lds_ldu(flagAddr) == 0u;
             const bool fresh = live && conv && !emitted;
-            const int any_fresh = __syncthreads_or(fresh ? 1 : 0);
-            if (any_fresh) {
+            if (__syncthreads_or(fresh ? 1 : 0)) {
+                // staging reuses the read copy of the converged codewords only
                 emit(fresh, t + 1);
                 if (fresh) emitted = true;
-                // the staging overwrote the read copy of converged codewords only
             }
             if (__syncthreads_and((emitted || !live) ? 1 : 0)) break;
         }

@@ -1,0 +1,117 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so) -- test infrastructure.
+
+The oracle is the checker only: tests compare the product library against it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+OR_BG1, OR_BG2, OR_WIFI = 1, 2, 3
+
+
+class OracleCode(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("Z", ctypes.c_int), ("ils", ctypes.c_int),
+                ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("rows_full", ctypes.c_int),
+                ("cols_full", ctypes.c_int), ("kb", ctypes.c_int), ("V", ctypes.POINTER(ctypes.c_int))]
+
+
+class Oracle:
+    def __init__(self, lib):
+        self.lib = lib
+        P = ctypes.c_void_p
+        lib.oracle_code_init.argtypes = [ctypes.POINTER(OracleCode), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.oracle_code_free.argtypes = [ctypes.POINTER(OracleCode)]
+        lib.oracle_decode.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      P, P, P, ctypes.c_int]
+        lib.oracle_decode_noms.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, P, P, P, P]
+        lib.oracle_noms_loss_grad.argtypes = [ctypes.POINTER(OracleCode), P, P, ctypes.c_int, ctypes.c_int,
+                                              P, P, P, P]
+        lib.oracle_noms_loss_grad.restype = ctypes.c_double
+        lib.oracle_syndrome.argtypes = [ctypes.POINTER(OracleCode), P]
+        lib.oracle_noise_stddev.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.oracle_noise_stddev.restype = ctypes.c_float
+        lib.oracle_llr.argtypes = [P, ctypes.c_int, ctypes.c_float, ctypes.c_int, P]
+        lib.oracle_gauss.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        lib.oracle_gauss.restype = ctypes.c_float
+        lib.oracle_info_bit_word.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]
+        lib.oracle_info_bit_word.restype = ctypes.c_uint32
+        lib.oracle_setup_ldpc.argtypes = [ctypes.c_int, ctypes.c_float, P]
+        lib.oracle_base_matrix.argtypes = [ctypes.c_int, ctypes.c_int, P]
+
+    def code(self, kind, Z, m_rows=0):
+        c = OracleCode()
+        if self.lib.oracle_code_init(ctypes.byref(c), kind, Z, m_rows) != 0:
+            raise ValueError((kind, Z, m_rows))
+        return c
+
+    def shifts(self, c):
+        """Dense V mod Z (-1 = zero block) restricted to the rows/columns in use."""
+        v = np.ctypeslib.as_array(c.V, shape=(c.rows_full * c.cols_full,)).reshape(c.rows_full, c.cols_full)
+        return v[:c.rows, :c.cols].copy()
+
+    def decode(self, c, llr, n_iter, early_exit=False, threads=8):
+        llr = np.ascontiguousarray(llr, dtype=np.float32)
+        n_cw = llr.shape[0]
+        nvn = c.cols * c.Z
+        assert llr.shape[1] == nvn
+        hd = np.empty((n_cw, nvn), dtype=np.uint8)
+        app = np.empty((n_cw, nvn), dtype=np.float32)
+        iters = np.empty(n_cw, dtype=np.int32)
+        self.lib.oracle_decode(ctypes.byref(c), llr.ctypes.data, n_cw, n_iter, int(early_exit), hd.ctypes.data,
+                               app.ctypes.data, iters.ctypes.data, threads)
+        return hd, app, iters
+
+    def decode_noms(self, c, llr, n_iter, alpha, beta):
+        llr = np.ascontiguousarray(llr, dtype=np.float32)
+        n_cw, nvn = llr.shape
+        hd = np.empty((n_cw, nvn), dtype=np.uint8)
+        app = np.empty((n_cw, nvn), dtype=np.float32)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float32)
+        beta = np.ascontiguousarray(beta, dtype=np.float32)
+        self.lib.oracle_decode_noms(ctypes.byref(c), llr.ctypes.data, n_cw, n_iter, alpha.ctypes.data,
+                                    beta.ctypes.data, hd.ctypes.data, app.ctypes.data)
+        return hd, app
+
+    def noms_loss_grad(self, c, llr, bits, n_iter, alpha, beta, grad=True):
+        llr = np.ascontiguousarray(llr, dtype=np.float32)
+        bits = np.ascontiguousarray(bits, dtype=np.uint8)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        beta = np.ascontiguousarray(beta, dtype=np.float64)
+        ga = np.zeros_like(alpha) if grad else None
+        gb = np.zeros_like(beta) if grad else None
+        loss = self.lib.oracle_noms_loss_grad(ctypes.byref(c), llr.ctypes.data, bits.ctypes.data, llr.shape[0],
+                                              n_iter, alpha.ctypes.data, beta.ctypes.data,
+                                              ga.ctypes.data if grad else None, gb.ctypes.data if grad else None)
+        return loss, ga, gb
+
+    def syndrome(self, c, hd_row):
+        hd_row = np.ascontiguousarray(hd_row, dtype=np.uint8)
+        return self.lib.oracle_syndrome(ctypes.byref(c), hd_row.ctypes.data)
+
+    def setup_ldpc(self, A, rate):
+        out = (ctypes.c_int * 10)()
+        self.lib.oracle_setup_ldpc(A, rate, out)
+        return list(out)
+
+    def base_matrix(self, kind, ils):
+        rows, cols = {1: (46, 68), 2: (42, 52), 3: (12, 24)}[kind]
+        out = np.empty(rows * cols, dtype=np.int32)
+        self.lib.oracle_base_matrix(kind, ils, out.ctypes.data)
+        return out.reshape(rows, cols)
+
+
+_ORACLE = None
+
+
+def load():
+    global _ORACLE
+    if _ORACLE is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
+        _ORACLE = Oracle(ctypes.CDLL(LIB))
+    return _ORACLE
