@@ -23,6 +23,28 @@ if not os.path.exists(LIB_PATH):
     raise ImportError(
         f"{LIB_PATH} is missing: build the HIP library first (python -m nr_gpu_ldpc_amd.build)")
 
+
+def _single_hip_runtime():
+    """Load the HIP runtime that PyTorch would load, before libnrldpc.so resolves its own.
+
+    PyTorch-ROCm ships a private libamdhip64 with the same soname as /opt/rocm's.  Whichever
+    library loads first decides; loading ours first would leave two HIP runtimes (two device
+    enumerations) in one process once torch is imported.  Without torch this is a no-op and
+    the system runtime is used.
+    """
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.origin:
+        return
+    hip = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(hip):
+        ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
+_single_hip_runtime()
 lib = ctypes.CDLL(LIB_PATH)
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)

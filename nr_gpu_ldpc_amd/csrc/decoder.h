@@ -9,7 +9,7 @@
 namespace nrldpc {
 
 constexpr int kMaxBlock = 512;          // threads per workgroup (2 waves per SIMD)
-constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per workgroup
+constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per CU (upper bound; see max_lds_bytes)
 
 // Per base-graph edge, in row-major edge order (two dwords, read through the scalar cache).
 // For an edge into an LDS-resident column (nd = block column j < NND):
@@ -34,6 +34,7 @@ struct DecodeArgs {
 };
 
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
+int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
 
 }  // namespace nrldpc

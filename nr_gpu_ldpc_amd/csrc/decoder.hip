@@ -362,15 +362,22 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
 
 template <int KIND, bool LEARNED>
 static int launch_one(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
-    static bool attr_set = false;
-    if (!attr_set) {
-        NRLDPC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_decode<KIND, LEARNED>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, kMaxLdsBytes));
-        attr_set = true;
-    }
     hipLaunchKernelGGL((k_decode<KIND, LEARNED>), dim3(blocks), dim3(threads), lds, s, a);
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;
+}
+
+// Dynamic LDS available to one workgroup on the current device (queried once per process).
+int max_lds_bytes() {
+    static int v = -1;
+    if (v < 0) {
+        int dev = 0, bytes = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&bytes, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            bytes = 64 * 1024;
+        v = bytes - 1024;  // headroom for the kernels' static LDS
+    }
+    return v;
 }
 
 size_t decoder_lds_bytes_per_cw(int kind, int Z) {
@@ -385,7 +392,10 @@ int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s) {
     const int blocks = (a.n_cw + a.cpb - 1) / a.cpb;
     const int threads = a.cpb * a.Z;
     const size_t lds = (size_t)a.cpb * a.lds_cw_bytes;
-    if (threads > kMaxBlock || lds > (size_t)kMaxLdsBytes || blocks < 1) return LDPC_EINVAL;
+    if (threads > kMaxBlock || lds > (size_t)max_lds_bytes() || blocks < 1) {
+        set_last_error("launch shape exceeds the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
     if (kind == kBG1) return learned ? launch_one<kBG1, true>(a, blocks, threads, lds, s)
                                      : launch_one<kBG1, false>(a, blocks, threads, lds, s);
     if (kind == kBG2) return learned ? launch_one<kBG2, true>(a, blocks, threads, lds, s)
