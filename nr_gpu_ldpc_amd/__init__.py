@@ -14,7 +14,8 @@ __all__ = ["lib", "Code", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "s
            "base_matrix", "noise_stddev", "generator_shifts", "device_count"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libnrldpc.so")
+# NRLDPC_LIB selects another build of the same library (used to compare build variants).
+LIB_PATH = os.environ.get("NRLDPC_LIB") or os.path.join(HERE, "lib", "libnrldpc.so")
 
 BG1, BG2, QC12X24 = 1, 2, 3
 EARLY_EXIT = 1

@@ -14,8 +14,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-BUILD = os.path.join(HERE, "_build")
-LIBDIR = os.path.join(HERE, "lib")
+BUILD = os.environ.get("NRLDPC_BUILD_DIR") or os.path.join(HERE, "_build")
+LIBDIR = os.environ.get("NRLDPC_LIB_DIR") or os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnrldpc.so")
 ROOT = os.path.dirname(HERE)
 
@@ -34,9 +34,9 @@ COMMON = [
     "-Wno-unused-command-line-argument",
     f"-I{os.path.join(ROOT, 'include')}",
     f"-I{CSRC}",
-]
+] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
-SOURCES = ["decoder.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
+SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
            "encoder.cpp", "compat.cpp", "driver.cpp", "autotune.cpp", "mixed.hip"]
 
 

@@ -50,7 +50,7 @@ int default_cpb(const CodeDesc& c) {
     int best = 1;
     double best_eff = 0.0;
     const size_t lim = (size_t)max_lds_bytes();
-    for (int cpb = 1; cpb * c.Z <= kMaxBlock && cpb * lds <= lim; cpb++) {
+    for (int cpb = 1; cpb * c.Z <= kMaxBlock && cpb * lds + 16 <= lim; cpb++) {
         const int th = cpb * c.Z;
         const double eff = (double)th / (double)(((th + 63) / 64) * 64);
         if (eff > best_eff + 1e-9) { best_eff = eff; best = cpb; }
@@ -203,7 +203,7 @@ int ldpc_set_launch(ldpc_ctx* x, int cpb) {
     if (!x || cpb < 0) return LDPC_EINVAL;
     if (cpb == 0) { x->cpb = x->cpb_default; return LDPC_OK; }
     const size_t lds = decoder_lds_bytes_per_cw(x->code.kind, x->code.Z);
-    if (cpb * x->code.Z > kMaxBlock || cpb * lds > (size_t)max_lds_bytes()) {
+    if (cpb * x->code.Z > kMaxBlock || cpb * lds + 16 > (size_t)max_lds_bytes()) {
         set_last_error("cw_per_block exceeds the workgroup size or LDS limit");
         return LDPC_EINVAL;
     }

@@ -31,6 +31,7 @@ struct DecodeArgs {
     int n_cw, Z, m_rows, n_vn, hd_bytes, n_iter, cpb, early_exit;
     int lds_cw_bytes;
     int nnz;
+    int force_generic;     // 1: use the any-Z kernel even where a specialised one exists
 };
 
 size_t decoder_lds_bytes_per_cw(int kind, int Z);

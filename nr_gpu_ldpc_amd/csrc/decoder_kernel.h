@@ -1,0 +1,441 @@
+// decoder_kernel.h -- the fused decoder kernel template (included by the per-code
+// instantiation units decoder_bg1.hip, decoder_bg2.hip, decoder_qc.hip).
+#pragma once
+// decoder.hip -- fused flooding (learned) scaled min-sum decoder for gfx950.
+//
+// One workgroup decodes `cpb` codewords; lane (cw, r) owns expanded row r of every block row
+// and sub-column r of every block column of codeword cw.  All iterations run inside one
+// launch with the whole decoder state on chip:
+//
+//   LDS, per codeword   APP of the block columns of degree > 1, twice:
+//                       * read copy  [NND][2Z]  (sub-columns 0..Z-1 stored twice so the
+//                         circulant read at (r + V) never wraps: one VALU add per edge)
+//                       * accumulator [NND][Z]  (APP_new, updated in place)
+//   VGPRs, per lane     check-node state of every block row: the two scaled minima
+//                       (|R| values), the sign bits of R and the arg-min index
+//                       (12 bytes/row instead of the reference's R per edge), and the APP
+//                       of every degree-1 (extension parity) column the lane's row feeds.
+//
+// Arithmetic follows the reference kernels exactly (cnp_kernel_1st_iter / cnp_kernel /
+// vnp_kernel_normal / vnp_kernel_last_iter, src/ldpc_kernels.cu:72-468):
+//   Q = APP_old - R_old;  two-min with the 1000.0f start value;  R = +-fl(0.75 m);
+//   dt = fl(R_new - R_old);  APP_new = fl(...fl(APP_old + dt_i0) + dt_i1 ...) over incident
+//   block rows in ascending order;  HD = APP > 0 ? 0 : 1.
+// The arg-min index is replaced by the equality test |Q_k| == m1, which selects the same R
+// (a tie at the minimum forces m2 == m1), and (m1, m2) are updated with min / med3, which
+// reproduce the `<` / `else if <` rule exactly.  Per-column update order is kept by
+// processing block rows in ascending order and placing a workgroup barrier before the VN
+// update of every block row that shares an LDS column with the rows since the previous
+// barrier (compile-time "epochs").
+#include "code.h"
+#include "decoder.h"
+
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+#include <utility>
+
+#ifndef NRLDPC_LDS_ATOMIC_ACC
+#define NRLDPC_LDS_ATOMIC_ACC 0
+#endif
+
+namespace nrldpc {
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
+__device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
+
+extern __shared__ __attribute__((aligned(16))) unsigned char g_smem[];
+
+// LDS accesses through address-space-3 pointers made from 32-bit byte addresses: the
+// address goes straight into the ds_read/ds_write VGPR (no per-access base add).  Every
+// address below already includes lds_base(), the offset of the dynamic LDS block.
+using lds_f32 = __attribute__((address_space(3))) float;
+using lds_u32 = __attribute__((address_space(3))) uint32_t;
+using lds_u8 = __attribute__((address_space(3))) uint8_t;
+__device__ __forceinline__ uint32_t lds_base() {
+    return (uint32_t)(uintptr_t)(__attribute__((address_space(3))) unsigned char*)g_smem;
+}
+__device__ __forceinline__ float lds_ld(uint32_t a) { return *(lds_f32*)(uintptr_t)a; }
+__device__ __forceinline__ void lds_st(uint32_t a, float v) { *(lds_f32*)(uintptr_t)a = v; }
+__device__ __forceinline__ uint32_t lds_ldu(uint32_t a) { return *(lds_u32*)(uintptr_t)a; }
+__device__ __forceinline__ void lds_stu(uint32_t a, uint32_t v) { *(lds_u32*)(uintptr_t)a = v; }
+__device__ __forceinline__ void lds_stb(uint32_t a, uint8_t v) { *(lds_u8*)(uintptr_t)a = v; }
+
+// VN update of one LDS-resident column: APP_new += dt.  Within one epoch every column is
+// updated by at most one lane, so a plain read-modify-write is race free.
+__device__ __forceinline__ void lds_acc(uint32_t a, float dt) { lds_st(a, lds_ld(a) + dt); }
+
+// Workgroup-wide "any lane true" through one LDS word (no static LDS, so the dynamic LDS
+// region starts at address 0 and every offset folds into the instruction).
+__device__ __forceinline__ bool block_any(bool v, uint32_t slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) lds_stu(slot, 0u);
+    __syncthreads();
+    if (v) lds_stu(slot, 1u);
+    __syncthreads();
+    const bool any = lds_ldu(slot) != 0u;
+    __syncthreads();
+    return any;
+}
+
+// Per-edge table entries are read through the constant address space so that the compiler
+// issues scalar (s_load) reads of these wave-uniform values.
+using CEdge = const EdgeParam __attribute__((address_space(4)));
+
+template <int KIND>
+constexpr bool nd_columns_leading() {
+    using S = Structure<KIND>;
+    for (int j = 0; j < S::NND; j++)
+        if (S::T.nd_index[j] != j) return false;
+    return true;
+}
+
+
+constexpr int lifting_set_constexpr(int Z) {
+    const int base[8] = {2, 3, 5, 7, 9, 11, 13, 15};
+    for (int i = 0; i < 8; i++)
+        for (int z = base[i]; z <= 384; z *= 2)
+            if (z == Z) return i;
+    return -1;
+}
+
+// V mod Z of edge e of base graph KIND at lifting size ZC (compile time; ZC > 0).
+template <int KIND, int ZC>
+constexpr int edge_vm(int e) {
+    if constexpr (KIND == kBG1) return kBG1Entries[e].v[lifting_set_constexpr(ZC)] % ZC;
+    else if constexpr (KIND == kBG2) return kBG2Entries[e].v[lifting_set_constexpr(ZC)] % ZC;
+    else return kWifiEntries[e].v % ZC;
+}
+
+
+// Read copy of block column J (< NND) at sub-column (r + V) for edge E of this lane.
+template <int ZC, int KIND, int E, int J, class ET>
+__device__ __forceinline__ float read_copy(uint32_t rRd, uint32_t rRdHi, ET et) {
+    if constexpr (ZC > 0) {
+        constexpr uint32_t off = (uint32_t)(J * 2 * ZC + edge_vm<KIND, ZC>(E)) * 4u;
+        if constexpr (off < 0x10000u) return lds_ld(rRd + off);
+        else return lds_ld(rRdHi + (off - 0x10000u));
+    } else {
+        return lds_ld(rRd + et[E].off_rd);
+    }
+}
+
+// Accumulator address of block column J (< NND), sub-column (r + V) mod Z, for edge E.
+template <int ZC, int KIND, int E, int J, class ET>
+__device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc, uint32_t rAccW, ET et) {
+    if constexpr (ZC > 0) {
+        constexpr int vm = edge_vm<KIND, ZC>(E);
+        constexpr uint32_t off = (uint32_t)(J * ZC + vm) * 4u;
+        static_assert(off < 0x10000u, "accumulator offset exceeds the LDS immediate");
+        if constexpr (vm == 0) return rAcc + off;
+        else return (r >= ZC - vm ? rAccW : rAcc) + off;
+    } else {
+        const uint32_t acc = et[E].acc;
+        return (rHi >= acc ? rAccW : rAcc) + (acc & 0xFFFFu);
+    }
+}
+
+// ZC > 0: kernel specialised for one lifting size, every circulant offset is an immediate.
+// ZC == 0: any lifting size, offsets come from the per-edge table (scalar loads).
+template <int KIND, int ZC, bool LEARNED>
+__global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
+    using S = Structure<KIND>;
+    static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
+    constexpr int ROWS = S::ROWS;
+    constexpr int NND = S::NND;
+    constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
+
+    const int Z = ZC > 0 ? ZC : a.Z;
+    const int tid = threadIdx.x;
+    const int cwl = tid / Z;
+    const int r = tid - cwl * Z;
+    const int cw = blockIdx.x * a.cpb + cwl;
+    const bool live = cw < a.n_cw;
+    const int cwc = live ? cw : a.n_cw - 1;
+    const uint32_t Z4 = 4u * Z;
+    const uint32_t base = lds_base() + cwl * a.lds_cw_bytes;
+    uint32_t rRd = base + 4u * r;                  // read copy, block column 0
+    uint32_t rAcc = base + 2u * NND * Z4 + 4u * r; // accumulator, block column 0
+    uint32_t rAccW = rAcc - Z4;                    // same, for lanes whose (r+V) wraps
+    uint32_t rRdHi = rRd + 0x10000u;               // second base: immediates are 16-bit
+    const uint32_t rHi = ((uint32_t)r << 16) | 0xFFFFu;  // compared with EdgeParam::acc
+    const uint32_t flagAddr = base + 3u * NND * Z4;
+    const uint32_t wgFlag = lds_base() + (uint32_t)a.cpb * a.lds_cw_bytes;  // workgroup flag slot
+    CEdge* et = (CEdge*)(a.etab);
+    const int m_rows = a.m_rows;
+
+    constexpr int NW = S::NW;
+    float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
+    uint32_t W[NW];            // per row: sign of R per edge + arg-min index (see Structure)
+    float app1[ND1];           // APP of the degree-1 columns fed by this lane's rows
+#pragma unroll
+    for (int i = 0; i < ROWS; i++) { A1[i] = 0.f; A2[i] = 0.f; }
+#pragma unroll
+    for (int i = 0; i < NW; i++) W[i] = 0u;
+
+    // ---- load channel LLRs
+    const float* __restrict__ L = a.llr + (size_t)cwc * a.n_vn;
+#pragma unroll 2
+    for (int j = 0; j < NND; j++) {
+        const float v = L[j * Z + r] + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
+        lds_st(rRd + j * 2 * Z4, v);
+        lds_st(rRd + j * 2 * Z4 + Z4, v);
+        lds_st(rAcc + j * Z4, v);
+    }
+    static_for<NND, S::COLS>([&](auto JC) {
+        constexpr int j = JC;
+        constexpr int e = S::T.d1_edge[j];
+        constexpr int i = S::T.d1_row[j];
+        if (i < m_rows) {
+            int c;
+            if constexpr (ZC > 0) c = r + edge_vm<KIND, ZC>(e);
+            else c = r + (int)et[e].off_rd;
+            if (c >= Z) c -= Z;
+            app1[S::T.d1_slot[j]] = L[j * Z + c] + 0.0f;
+        }
+    });
+    __syncthreads();
+
+    bool emitted = false;
+    const bool early = a.early_exit != 0;
+    const float2* __restrict__ wt = a.weights;  // learned: [(n_iter+1)][nnz] (alpha, beta), row 0 zero
+
+    // Writes hard decisions (and APP) of every lane's codeword for which `mine` holds.
+    auto emit = [&](bool mine, int iters) {
+        if (mine) {
+            // Loop-variant copies: no address of this path is precomputed outside it.
+            uint32_t eb = base;
+            int er = r;
+            float* ap_out = a.app_out;
+            CEdge* ee = et;
+            asm volatile("" : "+v"(eb), "+v"(er), "+s"(ap_out), "+s"(ee));
+            if (ap_out) ap_out += (size_t)cw * a.n_vn;
+#pragma unroll 1
+            for (int j = 0; j < NND; j++) {
+                const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                lds_stb(eb + j * Z + er, x > 0.f ? 0 : 1);
+                if (ap_out) ap_out[j * Z + er] = x;
+            }
+            static_for<NND, S::COLS>([&](auto JC) {
+                constexpr int j = JC;
+                constexpr int e = S::T.d1_edge[j];
+                constexpr int i = S::T.d1_row[j];
+                if (i < m_rows) {
+                    int c;
+                    if constexpr (ZC > 0) c = er + edge_vm<KIND, ZC>(e);
+                    else c = er + (int)ee[e].off_rd;
+                    if (c >= Z) c -= Z;
+                    const float x = app1[S::T.d1_slot[j]];
+                    lds_stb(eb + j * Z + c, x > 0.f ? 0 : 1);
+                    if (ap_out) ap_out[j * Z + c] = x;
+                }
+            });
+            if (er < 8) lds_stb(eb + a.n_vn + er, 0);
+        }
+        __syncthreads();
+        if (mine) {
+            for (int b = r; b < a.hd_bytes; b += Z) {
+                const uint32_t v0 = lds_ldu(base + 8u * b), v1 = lds_ldu(base + 8u * b + 4u);
+                const uint32_t lo = (v0 | (v0 >> 7) | (v0 >> 14) | (v0 >> 21)) & 0xFu;
+                const uint32_t hi = (v1 | (v1 >> 7) | (v1 >> 14) | (v1 >> 21)) & 0xFu;
+                a.hd[(size_t)cw * a.hd_bytes + b] = (uint8_t)(lo | (hi << 4));
+            }
+            if (r == 0 && a.iters_out) a.iters_out[cw] = iters;
+        }
+        __syncthreads();
+    };
+
+    int t = 0;
+    for (; t < a.n_iter; ++t) {
+        // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
+        // out of the iteration loop into long-lived VGPRs.
+        asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW), "+v"(rRdHi));
+        const float2* __restrict__ w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
+        const float2* __restrict__ w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
+        static_for<0, ROWS>([&](auto IC) {
+            constexpr int i = IC;
+            constexpr int D = S::deg(i);
+            constexpr int E0 = S::T.row_start[i];
+            constexpr int WW = S::T.w_word[i];
+            constexpr int WO = S::T.w_off[i];                 // sign bits start here
+            constexpr int IO = S::half(i) ? WO + 11 : 24;     // arg-min index starts here
+            int mr = m_rows;
+            asm volatile("" : "+s"(mr));
+            if (i >= mr) return;
+            // Keep this row's table reads and wrap tests inside the row (bounded SGPR live
+            // ranges): both are loop invariant and would otherwise be hoisted and spilled.
+            asm volatile("" : "+s"(et));
+            int rr = r;
+            uint32_t rrHi = rHi;
+            asm volatile("" : "+v"(rr), "+v"(rrHi));
+            float q[D], ro[D];
+            float m1 = 1000.0f, m2 = 1000.0f;
+            uint32_t neg = 0u;
+            const float a1 = A1[i], a2 = A2[i];
+            const uint32_t wo = W[WW];
+            uint32_t idxo = (wo >> IO) & 31u;
+            asm volatile("" : "+v"(idxo));
+            // pass 1: Q = APP_old - R_old, two-min, signs
+            static_for<0, D>([&](auto KC) {
+                constexpr int k = KC;
+                constexpr int e = E0 + k;
+                constexpr int j = S::T.ecol[e];
+                float x;
+                if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                else x = app1[S::T.d1_slot[j]];
+                const float mo = (idxo == (uint32_t)k) ? a2 : a1;
+                float rv;
+                if constexpr (LEARNED) {
+                    const float2 w = w_old[e];
+                    const float mag = w.x * fmaxf(mo - w.y, 0.0f);
+                    rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
+                } else {
+                    rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
+                }
+                const float qv = x - rv;
+                q[k] = qv;
+                ro[k] = rv;
+                const float aq = fabsf(qv);
+                m2 = __builtin_amdgcn_fmed3f(m1, m2, aq);
+                m1 = fminf(m1, aq);
+                // sign bit of Q; edge k lands at bit D-1-k.  APP is never -0.0 (the channel
+                // LLRs are canonicalised on load and x + y = -0.0 needs x = y = -0.0), so Q is
+                // never -0.0 and the raw sign bit equals the reference's (Q < 0).
+                neg = __builtin_amdgcn_alignbit(neg, as_u(qv), 31);
+            });
+            const uint32_t sgn = __popc(neg) & 1u;
+            const uint32_t sbit = sgn << 31;
+            float n1, n2;
+            if constexpr (LEARNED) { n1 = m1; n2 = m2; }
+            else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
+            const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
+            // pass 2: R_new = (product of the other signs) * scaled min, dt = R_new - R_old
+            uint32_t idxn = 0u;
+            float dtv[D];
+            static_for<0, D>([&](auto KC) {
+                constexpr int k = KC;
+                constexpr int e = E0 + k;
+                const bool eq = fabsf(q[k]) == m1;
+                idxn = eq ? (uint32_t)k : idxn;
+                float ms;
+                if constexpr (LEARNED) {
+                    const float2 w = w_new[e];
+                    const float mag = w.x * fmaxf((eq ? n2 : n1) - w.y, 0.0f);
+                    ms = as_f(as_u(mag) ^ sbit);
+                } else {
+                    ms = eq ? n2s : n1s;
+                }
+                const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
+                dtv[k] = rn - ro[k];
+            });
+            A1[i] = n1;
+            A2[i] = n2;
+            constexpr uint32_t dmask = (1u << D) - 1u;
+            const uint32_t fresh_w = ((sgn ? ~neg : neg) & dmask) | (idxn << (IO - WO));
+            if constexpr (!S::half(i)) W[WW] = fresh_w;
+            else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
+            else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
+            if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
+            // VN update, ascending block row per column
+#if NRLDPC_LDS_ATOMIC_ACC
+            static_for<0, D>([&](auto KC) {
+                constexpr int k = KC;
+                constexpr int e = E0 + k;
+                constexpr int j = S::T.ecol[e];
+                if constexpr (j < NND) {
+                    const uint32_t addr = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                    __hip_atomic_fetch_add((lds_f32*)(uintptr_t)addr, dtv[k], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                } else {
+                    float& ap = app1[S::T.d1_slot[j]];
+                    ap = ap + dtv[k];
+                }
+            });
+#else
+            // read-modify-write in chunks of 8 edges: loads first, then stores
+            static_for<0, (D + 7) / 8>([&](auto CC) {
+                constexpr int C0 = CC * 8;
+                constexpr int CN = (D - C0) < 8 ? (D - C0) : 8;
+                uint32_t addr[CN];
+                float cur[CN];
+                static_for<0, CN>([&](auto KC) {
+                    constexpr int k = C0 + KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    if constexpr (j < NND) {
+                        addr[KC] = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                        cur[KC] = lds_ld(addr[KC]);
+                    }
+                });
+                static_for<0, CN>([&](auto KC) {
+                    constexpr int k = C0 + KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    if constexpr (j < NND) {
+                        lds_st(addr[KC], cur[KC] + dtv[k]);
+                    } else {
+                        float& ap = app1[S::T.d1_slot[j]];
+                        ap = ap + dtv[k];
+                    }
+                });
+            });
+#endif
+        });
+        __syncthreads();
+        if (t + 1 == a.n_iter) break;
+        // APP_new becomes the read copy of the next iteration.
+#pragma unroll 2
+        for (int j = 0; j < NND; j++) {
+            const float v = lds_ld(rAcc + j * Z4);
+            lds_st(rRd + j * 2 * Z4, v);
+            lds_st(rRd + j * 2 * Z4 + Z4, v);
+        }
+        if (early && r == 0) lds_stu(flagAddr, 0u);
+        __syncthreads();
+        if (early) {
+            // syndrome of the current hard decision; any unsatisfied check keeps decoding
+            uint32_t bad = 0u;
+            static_for<0, ROWS>([&](auto IC) {
+                constexpr int i = IC;
+                constexpr int D = S::deg(i);
+                constexpr int E0 = S::T.row_start[i];
+                int mr = m_rows;
+                asm volatile("" : "+s"(mr));
+                if (i >= mr) return;
+                asm volatile("" : "+s"(et));
+                uint32_t p = 0u;
+                static_for<0, D>([&](auto KC) {
+                    constexpr int e = E0 + KC;
+                    constexpr int j = S::T.ecol[e];
+                    float x;
+                    if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    else x = app1[S::T.d1_slot[j]];
+                    p ^= x > 0.f ? 0u : 1u;
+                });
+                bad |= p;
+            });
+            if (bad) lds_stu(flagAddr, 1u);
+            __syncthreads();
+            const bool conv = lds_ldu(flagAddr) == 0u;
+            const bool fresh = live && conv && !emitted;
+            if (block_any(fresh, wgFlag)) {
+                // staging reuses the read copy of the converged codewords only
+                emit(fresh, t + 1);
+                if (fresh) emitted = true;
+            }
+            if (!block_any(live && !emitted, wgFlag)) break;
+        }
+    }
+    if (block_any(live && !emitted, wgFlag))
+        emit(live && !emitted, t < a.n_iter ? t + 1 : a.n_iter);
+}
+
+}  // namespace nrldpc

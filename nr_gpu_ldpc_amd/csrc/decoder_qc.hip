@@ -1,0 +1,19 @@
+// decoder_qc.hip -- instantiations of the fused decoder for one code kind.
+#include "status.h"
+#include "decoder_kernel.h"
+
+namespace nrldpc {
+
+template <int KIND, int ZC, bool LEARNED>
+int launch_decode(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_decode<KIND, ZC, LEARNED>), dim3(blocks), dim3(threads), lds, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
+template int launch_decode<kWifi, 0, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int launch_decode<kWifi, 0, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int launch_decode<kWifi, 96, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int launch_decode<kWifi, 96, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
+
+}  // namespace nrldpc

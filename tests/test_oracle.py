@@ -1,0 +1,199 @@
+"""The CPU oracle pinned against things other than itself (CPU only).
+
+* Philox4x32-10 against the Random123 known-answer vectors.
+* The Gaussian stream by its moments.
+* The decoder against hand-derived closed forms and against an independent dense-matrix
+  numpy restatement of the reference kernels (first-strict-minimum index rule, float32).
+* The learned decoder: alpha=0.75/beta=0 is the plain decoder bit for bit; the analytic
+  gradient agrees with double-precision finite differences.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests.oracle_binding import OR_BG1, OR_BG2, OR_WIFI
+
+F = np.float32
+
+
+def test_philox_known_answers(oracle):
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        c = (ctypes.c_uint32 * 4)(*ctr)
+        k = (ctypes.c_uint32 * 2)(*key)
+        o = (ctypes.c_uint32 * 4)()
+        oracle.lib.oracle_philox4x32_10(c, k, o)
+        assert list(o) == want
+
+
+def test_gaussian_moments(oracle):
+    g = np.array([oracle.lib.oracle_gauss(5, 17, i) for i in range(200000)])
+    assert abs(g.mean()) < 0.01
+    assert abs(g.var() - 1) < 0.01
+    assert abs(((g - g.mean()) ** 4).mean() / g.var() ** 2 - 3) < 0.05
+    assert abs((np.abs(g) < 1).mean() - 0.6827) < 0.005
+
+
+def test_info_bits_balanced(oracle):
+    words = np.array([oracle.lib.oracle_info_bit_word(3, 9, w) for w in range(20000)], dtype=np.uint32)
+    bits = np.unpackbits(words.view(np.uint8))
+    assert abs(bits.mean() - 0.5) < 0.005
+
+
+def _dense_H(oracle, code):
+    V = oracle.shifts(code)
+    Z = code.Z
+    rows, cols = V.shape
+    H = np.zeros((rows * Z, cols * Z), dtype=np.uint8)
+    for i in range(rows):
+        for j in range(cols):
+            if V[i, j] >= 0:
+                for r in range(Z):
+                    H[i * Z + r, j * Z + (r + V[i, j]) % Z] = 1
+    return H
+
+
+def dense_minsum(H, llr, n_iter):
+    """Flooding scaled min-sum on a dense H, written from the reference kernels' arithmetic."""
+    M, N = H.shape
+    app = llr.astype(F).copy()
+    R = np.zeros((M, N), dtype=F)
+    nbr = [np.nonzero(H[m])[0] for m in range(M)]
+    for t in range(n_iter):
+        dt = np.zeros((M, N), dtype=F)
+        for m in range(M):
+            cols = nbr[m]
+            Q = app[cols] if t == 0 else (app[cols] - R[m, cols]).astype(F)
+            absq = np.abs(Q)
+            vals = np.concatenate([absq, [F(1000.0), F(1000.0)]]).astype(F)
+            idx = int(np.argmin(absq)) if absq.min() < F(1000.0) else 0  # first strict minimum
+            srt = np.sort(vals)
+            m1, m2 = srt[0], srt[1]
+            sq = np.where(Q < 0, F(-1.0), F(1.0))
+            sign = F(np.prod(sq))
+            mag = np.where(np.arange(len(cols)) == idx, m2, m1).astype(F)
+            Rn = ((F(0.75) * sign) * sq * mag).astype(F)
+            dt[m, cols] = Rn if t == 0 else (Rn - R[m, cols]).astype(F)
+            R[m, cols] = Rn
+        for m in range(M):  # ascending block row per column
+            cols = nbr[m]
+            app[cols] = (app[cols] + dt[m, cols]).astype(F)
+    return (app <= 0).astype(np.uint8), app
+
+
+@pytest.mark.parametrize("kind,Z,m", [(OR_WIFI, 5, 0), (OR_BG2, 2, 0), (OR_BG1, 3, 0), (OR_BG2, 3, 6)])
+def test_oracle_matches_dense_restatement(oracle, kind, Z, m):
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    rng = np.random.default_rng(Z * 31 + kind)
+    n = H.shape[1]
+    llr = (rng.standard_normal((3, n)) * 2.5 + 1.0).astype(F)
+    llr[:, :2 * Z] = 0 if kind != OR_WIFI else llr[:, :2 * Z]
+    llr[2] = np.round(llr[2])  # ties
+    hd, app, _ = oracle.decode(code, llr, 5)
+    for w in range(3):
+        dhd, dapp = dense_minsum(H, llr[w], 5)
+        assert np.array_equal(hd[w], dhd)
+        assert np.array_equal(app[w], dapp)
+
+
+def _row_sets(oracle, code):
+    H = _dense_H(oracle, code)
+    return H, H.sum(axis=0)
+
+
+def test_closed_form_single_negative(oracle):
+    """All LLRs +5 except one column at -2, one iteration, Z=1 (the base matrix itself)."""
+    code = oracle.code(OR_WIFI, 1)
+    H, deg = _row_sets(oracle, code)
+    n = H.shape[1]
+    c0 = 4
+    llr = np.full((1, n), 5.0, dtype=F)
+    llr[0, c0] = -2.0
+    hd, app, _ = oracle.decode(code, llr, 1)
+    for c in range(n):
+        rows = np.nonzero(H[:, c])[0]
+        if c == c0:
+            want = -2.0 + 3.75 * len(rows)                 # other signs +, m2 = 5 -> 0.75*5
+        else:
+            want = 5.0 + sum(-1.5 if H[m, c0] else 3.75 for m in rows)  # min 2 with a - sign
+        assert app[0, c] == F(want)
+
+
+def test_closed_form_clamp(oracle):
+    """Every |LLR| above the 1000.0f start value: every message is 0.75 * 1000."""
+    code = oracle.code(OR_WIFI, 3)
+    H, deg = _row_sets(oracle, code)
+    llr = np.full((1, H.shape[1]), 2000.0, dtype=F)
+    _, app, _ = oracle.decode(code, llr, 1)
+    assert np.array_equal(app[0], (2000.0 + 750.0 * deg).astype(F))
+
+
+@pytest.mark.parametrize("kind,Z", [(OR_BG1, 8), (OR_BG2, 6), (OR_WIFI, 4)])
+def test_noiseless_codeword_one_iteration(oracle, nr, kind, Z):
+    code = nr.Code({OR_BG1: 1, OR_BG2: 2, OR_WIFI: 3}[kind], Z, 0, -1)
+    info = np.random.default_rng(Z).integers(0, 2, (2, code.K), dtype=np.uint8)
+    cw = code.encode_host(info)
+    llr = (10.0 * (1.0 - 2.0 * cw)).astype(F)
+    llr[:, :code.punct * Z] = 0.0
+    hd, _, _ = oracle.decode(oracle.code(kind, Z), llr, 1)
+    assert np.array_equal(hd, cw)
+
+
+def test_early_exit_stops_on_zero_syndrome(oracle, nr):
+    code = nr.Code(2, 16, 0, -1)
+    info = np.random.default_rng(1).integers(0, 2, (4, code.K), dtype=np.uint8)
+    cw = code.encode_host(info)
+    llr = (4.0 * (1.0 - 2.0 * cw)).astype(F)
+    llr[:, :2 * 16] = 0.0
+    oc = oracle.code(OR_BG2, 16)
+    hd, _, iters = oracle.decode(oc, llr, 30, early_exit=True)
+    assert (iters < 30).all() and np.array_equal(hd, cw)
+
+
+@pytest.mark.parametrize("kind,Z", [(OR_BG2, 6), (OR_WIFI, 5)])
+def test_noms_plain_weights_equal_plain_decoder(oracle, kind, Z):
+    code = oracle.code(kind, Z)
+    nnz = int((oracle.shifts(code) >= 0).sum())
+    rng = np.random.default_rng(2)
+    llr = (rng.standard_normal((3, code.cols * Z)) * 2 + 0.7).astype(F)
+    T = 6
+    a = np.full(T * nnz, 0.75, dtype=F)
+    b = np.zeros(T * nnz, dtype=F)
+    hd, app = oracle.decode_noms(code, llr, T, a, b)
+    phd, papp, _ = oracle.decode(code, llr, T)
+    assert np.array_equal(hd, phd) and np.array_equal(app.view(np.uint32), papp.view(np.uint32))
+
+
+def test_noms_gradient_matches_finite_differences(oracle, nr):
+    code = oracle.code(OR_BG2, 3)
+    nnz = int((oracle.shifts(code) >= 0).sum())
+    T = 3
+    rng = np.random.default_rng(4)
+    c = nr.Code(2, 3, 0, -1)
+    info = rng.integers(0, 2, (2, c.K), dtype=np.uint8)
+    bits = c.encode_host(info)
+    llr = ((1.0 - 2.0 * bits) * 1.2 + rng.standard_normal(bits.shape) * 1.1).astype(F)
+    llr[:, :6] = 0.0
+    alpha = rng.uniform(0.6, 0.9, T * nnz)
+    beta = rng.uniform(-0.2, 0.2, T * nnz)
+    loss, ga, gb = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta)
+    h = 1e-6
+    idx = rng.choice(T * nnz, 12, replace=False)
+    for i in idx:
+        for vec, g in ((alpha, ga), (beta, gb)):
+            v0 = vec[i]
+            vec[i] = v0 + h
+            lp, _, _ = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, grad=False)
+            vec[i] = v0 - h
+            lm, _, _ = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, grad=False)
+            vec[i] = v0
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(fd), (i, fd, g[i])
+    assert np.abs(ga).sum() > 0 and np.abs(gb).sum() > 0
