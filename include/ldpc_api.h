@@ -34,6 +34,7 @@ extern "C" {
 #define LDPC_CODE_QC12X24 3    /* 12 x 24 QC example code (802.11n rate-1/2 prototype), K = 12 Z */
 
 #define LDPC_FLAG_EARLY_EXIT 1u /* stop a codeword once its hard decision has zero syndrome */
+#define LDPC_FLAG_FALLBACK_PLAIN 2u /* learned decode: re-decode syndrome failures with plain min-sum */
 
 typedef struct ldpc_ctx ldpc_ctx;
 
@@ -48,6 +49,7 @@ typedef struct {
     int hd_bytes;      /* packed hard-decision bytes per codeword = ceil(n_vn / 8) */
     int nnz;           /* block edges in use */
     int cw_per_block;  /* codewords per workgroup used by the decoder launch */
+    int nnz_full;      /* block edges of the whole base graph (learned-weight row length) */
 } ldpc_code_info;
 
 const char* ldpc_strerror(int status);
@@ -113,6 +115,41 @@ int ldpc_encode_host(const ldpc_ctx* ctx, const uint8_t* info, int n_cw, uint8_t
  * src/Gen_shift_value.h. */
 int ldpc_generator_shifts(int kind, int Z, short* no_shift, short* pointer_shift, short* gen_shift,
                           int gen_len);
+
+/* ----------------------------------------------------------------- learned decoder
+ * Learned offset/normalized min-sum: R = alpha[t][e] * sign * max(m - beta[t][e], 0), one
+ * (alpha, beta) per iteration t and base-graph edge e (row-major edge order of the whole base
+ * graph, nnz_full per iteration).  alpha = 0.75, beta = 0 reproduces the plain decoder bit for
+ * bit.  No reference counterpart (the reference hard-codes alpha = 0.75,
+ * src/ldpc_kernels.cu:164,295). */
+
+/* Set the weights for n_iter iterations: host array [n_iter][nnz_full][2] = (alpha, beta). */
+int ldpc_noms_set_weights(ldpc_ctx* ctx, int n_iter, const float* weights);
+/* Copy the current weights to the host ([n_iter][nnz_full][2]); *n_iter receives T. */
+int ldpc_noms_get_weights(ldpc_ctx* ctx, int* n_iter, float* weights);
+
+/* Decode with the learned weights (n_iter must equal the weights' iteration count).  With
+ * LDPC_FLAG_FALLBACK_PLAIN, codewords whose learned hard decision fails the syndrome check are
+ * decoded again with plain min-sum (n_iter iterations); their outputs replace the learned ones.
+ * Outputs as for ldpc_decode_async. */
+int ldpc_decode_learned_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
+                              unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+
+/* Workspace bytes for ldpc_noms_train_grad_async with n_cw codewords per call. */
+size_t ldpc_noms_workspace_bytes(const ldpc_ctx* ctx, int n_cw, int n_iter);
+
+/* One gradient evaluation on the device: learned forward with training records, mean BCE of
+ * sigmoid(-APP) against cw_bits, hand-written backward.  grad [n_iter][nnz_full][2] and loss[1]
+ * (device float) are ACCUMULATED (clear them first).  workspace: device, at least
+ * ldpc_noms_workspace_bytes(ctx, n_cw, n_iter) bytes. */
+int ldpc_noms_train_grad_async(ldpc_ctx* ctx, void* stream, const float* llr, const uint8_t* cw_bits,
+                               int n_cw, int n_iter, float* grad, float* loss, void* workspace,
+                               size_t workspace_bytes);
+
+/* Adam update of the context's weights in place from grad [n_iter][nnz_full][2]; m and v are
+ * caller-owned device moment buffers of the same size (zero before step 1). */
+int ldpc_noms_adam_async(ldpc_ctx* ctx, void* stream, const float* grad, float* m, float* v, int step,
+                         float lr);
 
 /* ----------------------------------------------------------------- on-device channel */
 

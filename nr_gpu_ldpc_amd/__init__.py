@@ -19,6 +19,7 @@ LIB_PATH = os.environ.get("NRLDPC_LIB") or os.path.join(HERE, "lib", "libnrldpc.
 
 BG1, BG2, QC12X24 = 1, 2, 3
 EARLY_EXIT = 1
+FALLBACK_PLAIN = 2
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -54,7 +55,7 @@ _c_int_p = ctypes.POINTER(ctypes.c_int)
 class CodeInfo(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in
                 ("kind", "Z", "ils", "rows", "cols", "kb", "punct", "n_vn", "K", "N", "hd_bytes",
-                 "nnz", "cw_per_block")]
+                 "nnz", "cw_per_block", "nnz_full")]
 
 
 def _sig(name, res, *args):
@@ -83,6 +84,13 @@ _sig("ldpc_noise_stddev", ctypes.c_float, ctypes.c_float, ctypes.c_float)
 _sig("ldpc_channel_async", ctypes.c_int, _P, _P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _P,
      ctypes.c_float, _P, _P)
 _sig("ldpc_count_errors_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_noms_set_weights", ctypes.c_int, _P, ctypes.c_int, _P)
+_sig("ldpc_noms_get_weights", ctypes.c_int, _P, _c_int_p, _P)
+_sig("ldpc_decode_learned_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_noms_workspace_bytes", ctypes.c_size_t, _P, ctypes.c_int, ctypes.c_int)
+_sig("ldpc_noms_train_grad_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P,
+     ctypes.c_size_t)
+_sig("ldpc_noms_adam_async", ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_float)
 
 
 class LdpcError(RuntimeError):
@@ -187,6 +195,42 @@ class Code:
     def count_errors(self, hd_ptr, cw_bits_ptr, n_cw, counts_ptr, stream=0):
         _check(lib.ldpc_count_errors_async(self._h, stream, hd_ptr, cw_bits_ptr, n_cw, counts_ptr),
                "ldpc_count_errors_async")
+
+    # ---- learned offset/normalized min-sum
+    def set_weights(self, weights):
+        """weights: float32 [n_iter, nnz_full, 2] (alpha, beta)."""
+        import numpy as np
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        assert w.ndim == 3 and w.shape[1:] == (self.nnz_full, 2)
+        _check(lib.ldpc_noms_set_weights(self._h, w.shape[0], w.ctypes.data), "ldpc_noms_set_weights")
+
+    def get_weights(self):
+        import numpy as np
+        t = ctypes.c_int()
+        _check(lib.ldpc_noms_get_weights(self._h, ctypes.byref(t), None), "ldpc_noms_get_weights")
+        w = np.empty((t.value, self.nnz_full, 2), dtype=np.float32)
+        _check(lib.ldpc_noms_get_weights(self._h, ctypes.byref(t), w.ctypes.data), "ldpc_noms_get_weights")
+        return w
+
+    def plain_weights(self, n_iter):
+        import numpy as np
+        w = np.zeros((n_iter, self.nnz_full, 2), dtype=np.float32)
+        w[..., 0] = 0.75
+        return w
+
+    def decode_learned(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        _check(lib.ldpc_decode_learned_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,
+                                             app_ptr or None, iters_ptr or None), "ldpc_decode_learned_async")
+
+    def workspace_bytes(self, n_cw, n_iter):
+        return lib.ldpc_noms_workspace_bytes(self._h, n_cw, n_iter)
+
+    def train_grad(self, llr_ptr, bits_ptr, n_cw, n_iter, grad_ptr, loss_ptr, ws_ptr, ws_bytes, stream=0):
+        _check(lib.ldpc_noms_train_grad_async(self._h, stream, llr_ptr, bits_ptr, n_cw, n_iter, grad_ptr,
+                                              loss_ptr, ws_ptr, ws_bytes), "ldpc_noms_train_grad_async")
+
+    def adam(self, grad_ptr, m_ptr, v_ptr, step, lr, stream=0):
+        _check(lib.ldpc_noms_adam_async(self._h, stream, grad_ptr, m_ptr, v_ptr, step, lr), "ldpc_noms_adam_async")
 
     def encode_host(self, info_bits):
         """info_bits: uint8 array [n_cw, K] -> codeword bits uint8 [n_cw, n_vn] (host)."""

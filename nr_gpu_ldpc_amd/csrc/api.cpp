@@ -173,7 +173,7 @@ int ldpc_ctx_destroy(ldpc_ctx* x) {
     if (x->device < 0) { delete x; return LDPC_OK; }
     (void)hipSetDevice(x->device);
     void* ptrs[] = {x->d_etab, x->d_row_start, x->d_edge_col, x->d_edge_vm, x->d_step_target,
-                    x->d_step_start, x->d_term_col, x->d_term_shift, x->d_weights};
+                    x->d_step_start, x->d_term_col, x->d_term_shift, x->d_weights, x->d_scratch};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete x;
@@ -196,6 +196,7 @@ int ldpc_ctx_info(const ldpc_ctx* x, ldpc_code_info* info) {
     info->hd_bytes = (c.n_vn() + 7) / 8;
     info->nnz = c.nnz();
     info->cw_per_block = x->cpb;
+    info->nnz_full = nnz_full(c.kind);
     return LDPC_OK;
 }
 
@@ -215,7 +216,7 @@ int ldpc_decode_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int
                       uint8_t* hd_packed, float* app_out, int* iters_out) {
     if (!x || !llr || !hd_packed || n_cw < 0 || n_iter < 0) return LDPC_EINVAL;
     if (n_cw == 0) return LDPC_OK;
-    return decode_with(x, stream, llr, n_cw, n_iter, flags, hd_packed, app_out, iters_out, nullptr);
+    return decode_with(x, stream, llr, n_cw, n_iter, flags, hd_packed, app_out, iters_out, nullptr, nullptr);
 }
 
 int ldpc_syndrome_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, int* weight) {

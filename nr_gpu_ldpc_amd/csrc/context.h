@@ -28,6 +28,9 @@ struct ldpc_ctx {
     // learned offset/normalized min-sum weights: [(n_iter + 1)][nnz_full] float2, row 0 zero
     float2* d_weights = nullptr;
     int weights_iters = 0;
+    // grow-only scratch for internal temporaries (fallback syndrome weights, training HD)
+    void* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
     std::mutex mu;
 };
 
@@ -38,6 +41,6 @@ int n_nd(int kind);
 int default_cpb(const CodeDesc& c);
 // Shared decode path; weights != null selects the learned decoder with [(n_iter+1)][nnz] weights.
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
-                uint8_t* hd, float* app_out, int* iters_out, const float2* weights);
+                uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* select);
 
 }  // namespace nrldpc

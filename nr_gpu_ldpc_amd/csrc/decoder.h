@@ -32,10 +32,17 @@ struct DecodeArgs {
     int lds_cw_bytes;
     int nnz;
     int force_generic;     // 1: use the any-Z kernel even where a specialised one exists
+    // training records (learned forward with records): [n_cw][n_iter][rows_full][Z]
+    float* rec_m1;
+    float* rec_m2;
+    uint32_t* rec_w;       // Q sign bits (edge k at bit d-1-k) | arg-min << 20 | 2nd arg-min << 25
+    const int* select;     // [n_cw] or null: decode only codewords with select[cw] != 0
 };
 
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
 int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
+// Learned forward that also writes the training records (generic-Z kernel).
+int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s);
 
 }  // namespace nrldpc

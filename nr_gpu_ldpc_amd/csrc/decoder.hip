@@ -11,6 +11,9 @@ namespace nrldpc {
 template <int KIND, int ZC, bool LEARNED>
 int launch_decode(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 
+template <int KIND>
+int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
+
 // Lifting sizes with a dedicated kernel (all offsets compile-time): the hot configurations.
 constexpr int kBG1Special = 384, kBG2Special = 208, kQCSpecial = 96;
 
@@ -63,6 +66,23 @@ int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s) {
                        : launch_decode<kWifi, kQCSpecial, false>(a, blocks, threads, lds, s);
     return learned ? launch_decode<kWifi, 0, true>(a, blocks, threads, lds, s)
                    : launch_decode<kWifi, 0, false>(a, blocks, threads, lds, s);
+}
+
+}  // namespace nrldpc
+
+namespace nrldpc {
+
+int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s) {
+    const int blocks = (a.n_cw + a.cpb - 1) / a.cpb;
+    const int threads = a.cpb * a.Z;
+    const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;
+    if (threads > kMaxBlock || lds > (size_t)max_lds_bytes() || blocks < 1) {
+        set_last_error("launch shape exceeds the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
+    if (kind == kBG1) return launch_decode_train<kBG1>(a, blocks, threads, lds, s);
+    if (kind == kBG2) return launch_decode_train<kBG2>(a, blocks, threads, lds, s);
+    return launch_decode_train<kWifi>(a, blocks, threads, lds, s);
 }
 
 }  // namespace nrldpc

@@ -145,7 +145,10 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 
 // ZC > 0: kernel specialised for one lifting size, every circulant offset is an immediate.
 // ZC == 0: any lifting size, offsets come from the per-edge table (scalar loads).
-template <int KIND, int ZC, bool LEARNED>
+// TRAIN (learned only): also store, per iteration and expanded row, the two minima, the
+// first-strict-minimum and second-minimum edge indices and the Q sign bits, for the
+// backward pass (layout [cw][iteration][row][Z]).
+template <int KIND, int ZC, bool LEARNED, bool TRAIN = false>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     using S = Structure<KIND>;
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
@@ -158,7 +161,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     const int cwl = tid / Z;
     const int r = tid - cwl * Z;
     const int cw = blockIdx.x * a.cpb + cwl;
-    const bool live = cw < a.n_cw;
+    const bool live = cw < a.n_cw && (a.select == nullptr || a.select[cw] != 0);
     const int cwc = live ? cw : a.n_cw - 1;
     const uint32_t Z4 = 4u * Z;
     const uint32_t base = lds_base() + cwl * a.lds_cw_bytes;
@@ -171,6 +174,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     const uint32_t wgFlag = lds_base() + (uint32_t)a.cpb * a.lds_cw_bytes;  // workgroup flag slot
     CEdge* et = (CEdge*)(a.etab);
     const int m_rows = a.m_rows;
+    if (a.select != nullptr && !block_any(live, wgFlag)) return;  // nothing to decode here
 
     constexpr int NW = S::NW;
     float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
@@ -281,6 +285,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             uint32_t neg = 0u;
             const float a1 = A1[i], a2 = A2[i];
             const uint32_t wo = W[WW];
+            uint32_t a1i = 31u, a2i = 31u;  // TRAIN: arg-min / second arg-min (31 = none)
             uint32_t idxo = (wo >> IO) & 31u;
             asm volatile("" : "+v"(idxo));
             // pass 1: Q = APP_old - R_old, two-min, signs
@@ -304,6 +309,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                 q[k] = qv;
                 ro[k] = rv;
                 const float aq = fabsf(qv);
+                if constexpr (TRAIN) {
+                    const bool lt1 = aq < m1, lt2 = aq < m2;
+                    a2i = lt1 ? a1i : (lt2 ? (uint32_t)k : a2i);
+                    a1i = lt1 ? (uint32_t)k : a1i;
+                }
                 m2 = __builtin_amdgcn_fmed3f(m1, m2, aq);
                 m1 = fminf(m1, aq);
                 // sign bit of Q; edge k lands at bit D-1-k.  APP is never -0.0 (the channel
@@ -313,6 +323,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             });
             const uint32_t sgn = __popc(neg) & 1u;
             const uint32_t sbit = sgn << 31;
+            if constexpr (TRAIN) {
+                if (live) {
+                    const size_t ri = (((size_t)cw * a.n_iter + t) * ROWS + i) * (size_t)Z + r;
+                    a.rec_m1[ri] = m1;
+                    a.rec_m2[ri] = m2;
+                    a.rec_w[ri] = neg | (a1i << 20) | (a2i << 25);
+                }
+            }
             float n1, n2;
             if constexpr (LEARNED) { n1 = m1; n2 = m2; }
             else { n1 = 0.75f * m1; n2 = 0.75f * m2; }

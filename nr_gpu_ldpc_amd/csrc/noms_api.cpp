@@ -1,0 +1,184 @@
+// noms_api.cpp -- C ABI of the learned offset/normalized min-sum decoder.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/ldpc_api.h"
+#include "context.h"
+#include "decoder.h"
+#include "learned.h"
+#include "status.h"
+
+namespace nrldpc {
+
+// Grow-only device scratch owned by the context (syndrome weights of the fallback path).
+static int ctx_scratch(ldpc_ctx* x, size_t bytes, void** out) {
+    if (x->scratch_bytes < bytes) {
+        if (x->d_scratch) NRLDPC_HIP(hipFree(x->d_scratch));
+        x->d_scratch = nullptr;
+        x->scratch_bytes = 0;
+        NRLDPC_HIP(hipMalloc(&x->d_scratch, bytes));
+        x->scratch_bytes = bytes;
+    }
+    *out = x->d_scratch;
+    return LDPC_OK;
+}
+
+static size_t records_per_cw(const ldpc_ctx* x, int n_iter) {
+    const int rows_full = x->code.rows_full;
+    return (size_t)n_iter * rows_full * x->code.Z;
+}
+
+}  // namespace nrldpc
+
+using namespace nrldpc;
+
+extern "C" {
+
+int ldpc_noms_set_weights(ldpc_ctx* x, int n_iter, const float* w) {
+    if (!x || n_iter <= 0 || !w) return LDPC_EINVAL;
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    const int nnz = nnz_full(x->code.kind);
+    std::vector<float> h((size_t)(n_iter + 1) * nnz * 2, 0.f);
+    std::copy(w, w + (size_t)n_iter * nnz * 2, h.begin() + (size_t)nnz * 2);
+    std::lock_guard<std::mutex> lk(x->mu);
+    if (x->d_weights && x->weights_iters != n_iter) {
+        NRLDPC_HIP(hipFree(x->d_weights));
+        x->d_weights = nullptr;
+    }
+    if (!x->d_weights) NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&x->d_weights), h.size() * sizeof(float)));
+    NRLDPC_HIP(hipMemcpy(x->d_weights, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    x->weights_iters = n_iter;
+    return LDPC_OK;
+}
+
+int ldpc_noms_get_weights(ldpc_ctx* x, int* n_iter, float* w) {
+    if (!x || !n_iter) return LDPC_EINVAL;
+    *n_iter = x->weights_iters;
+    if (!x->d_weights) return LDPC_EINVAL;
+    if (w) {
+        const int nnz = nnz_full(x->code.kind);
+        NRLDPC_HIP(hipSetDevice(x->device));
+        NRLDPC_HIP(hipMemcpy(w, reinterpret_cast<float*>(x->d_weights) + (size_t)nnz * 2,
+                             (size_t)x->weights_iters * nnz * 2 * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return LDPC_OK;
+}
+
+int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
+                              uint8_t* hd, float* app_out, int* iters_out) {
+    if (!x || !llr || !hd || n_cw < 0) return LDPC_EINVAL;
+    if (!x->d_weights || n_iter != x->weights_iters) {
+        set_last_error("learned weights missing or for a different iteration count");
+        return LDPC_EINVAL;
+    }
+    if (n_cw == 0) return LDPC_OK;
+    int st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                         x->d_weights, nullptr);
+    if (st || !(flags & LDPC_FLAG_FALLBACK_PLAIN)) return st;
+    void* scratch = nullptr;
+    if ((st = ctx_scratch(x, (size_t)n_cw * sizeof(int), &scratch))) return st;
+    int* weight = static_cast<int*>(scratch);
+    if ((st = ldpc_syndrome_async(x, stream, hd, n_cw, weight))) return st;
+    // plain min-sum over the codewords whose syndrome weight is nonzero
+    return decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                       nullptr, weight);
+}
+
+size_t ldpc_noms_workspace_bytes(const ldpc_ctx* x, int n_cw, int n_iter) {
+    if (!x || n_cw <= 0 || n_iter <= 0) return 0;
+    const size_t rec = records_per_cw(x, n_iter) * (size_t)n_cw;
+    const size_t app = (size_t)n_cw * x->code.n_vn();
+    const size_t n_wg = (size_t)std::min(n_cw, 1024);
+    const size_t gapp = n_wg * 2 * (size_t)x->code.n_vn();
+    return (rec * 3 + app + gapp) * sizeof(float) + 256;
+}
+
+int ldpc_noms_train_grad_async(ldpc_ctx* x, void* stream, const float* llr, const uint8_t* cw_bits, int n_cw,
+                               int n_iter, float* grad, float* loss, void* ws, size_t ws_bytes) {
+    if (!x || !llr || !cw_bits || !grad || !loss || !ws || n_cw <= 0) return LDPC_EINVAL;
+    if (!x->d_weights || n_iter != x->weights_iters) {
+        set_last_error("learned weights missing or for a different iteration count");
+        return LDPC_EINVAL;
+    }
+    if (ws_bytes < ldpc_noms_workspace_bytes(x, n_cw, n_iter)) {
+        set_last_error("training workspace too small");
+        return LDPC_ENOMEM;
+    }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    const CodeDesc& c = x->code;
+    const size_t rec = records_per_cw(x, n_iter) * (size_t)n_cw;
+    float* p = static_cast<float*>(ws);
+    float* rec_m1 = p;
+    float* rec_m2 = p + rec;
+    uint32_t* rec_w = reinterpret_cast<uint32_t*>(p + 2 * rec);
+    float* app = p + 3 * rec;
+    float* gapp = app + (size_t)n_cw * c.n_vn();
+    std::vector<uint8_t> dummy;
+    // forward with records; hard decisions go to the tail of the workspace-free hd buffer
+    DecodeArgs a{};
+    a.llr = llr;
+    a.app_out = app;
+    a.etab = x->d_etab;
+    a.weights = x->d_weights;
+    a.n_cw = n_cw;
+    a.Z = c.Z;
+    a.m_rows = c.rows;
+    a.n_vn = c.n_vn();
+    a.hd_bytes = (c.n_vn() + 7) / 8;
+    a.n_iter = n_iter;
+    a.cpb = 1;
+    a.lds_cw_bytes = (int)decoder_lds_bytes_per_cw(c.kind, c.Z);
+    a.nnz = nnz_full(c.kind);
+    a.rec_m1 = rec_m1;
+    a.rec_m2 = rec_m2;
+    a.rec_w = rec_w;
+    void* hd_scratch = nullptr;
+    int st = ctx_scratch(x, (size_t)n_cw * a.hd_bytes, &hd_scratch);
+    if (st) return st;
+    a.hd = static_cast<uint8_t*>(hd_scratch);
+    if ((st = decoder_launch_train(c.kind, a, static_cast<hipStream_t>(stream)))) return st;
+    BackwardArgs b{};
+    b.app = app;
+    b.bits = cw_bits;
+    b.rec_m1 = rec_m1;
+    b.rec_m2 = rec_m2;
+    b.rec_w = rec_w;
+    b.weights = x->d_weights;
+    b.edge_col = x->d_edge_col;
+    b.edge_vm = x->d_edge_vm;
+    b.grad = grad;
+    b.loss = loss;
+    b.gapp = gapp;
+    b.n_cw = n_cw;
+    b.n_iter = n_iter;
+    b.Z = c.Z;
+    b.m_rows = c.rows;
+    b.n_vn = c.n_vn();
+    b.nnz = nnz_full(c.kind);
+    b.inv_norm = 1.0f / ((float)c.n_vn() * (float)n_cw);
+    return backward_launch(c.kind, b, std::min(n_cw, 1024), static_cast<hipStream_t>(stream));
+}
+
+int ldpc_noms_adam_async(ldpc_ctx* x, void* stream, const float* grad, float* m, float* v, int step, float lr) {
+    if (!x || !grad || !m || !v || step < 1 || !x->d_weights) return LDPC_EINVAL;
+    NRLDPC_HIP(hipSetDevice(x->device));
+    const int nnz = nnz_full(x->code.kind);
+    AdamArgs a{};
+    a.w = reinterpret_cast<float*>(x->d_weights) + (size_t)nnz * 2;
+    a.grad = grad;
+    a.m = m;
+    a.v = v;
+    a.n = x->weights_iters * nnz * 2;
+    a.lr = lr;
+    a.b1 = 0.9f;
+    a.b2 = 0.999f;
+    a.eps = 1e-8f;
+    a.b1t = (float)std::pow(0.9, step);
+    a.b2t = (float)std::pow(0.999, step);
+    return adam_launch(a, static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

@@ -428,8 +428,8 @@ double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8
                     for (int k = 0; k < d; k++) {
                         const int e = e0 + k;
                         const int src = (k != a1) ? a1 : a2; /* edge whose |Q| is m */
-                        if (src < 0) continue;               /* 1000.0 clamp: constant m */
-                        const double m = fabs(Qt[(e0 + src) * Z + r]);
+                        /* src < 0: the 1000.0 start value is the minimum, a constant m */
+                        const double m = (src < 0) ? 1000.0 : fabs(Qt[(e0 + src) * Z + r]);
                         const int b = (t - 1) * E + e;
                         const double off = m - beta[b];
                         const double s = ((neg ^ (Qt[e * Z + r] < 0)) ? -1.0 : 1.0);
@@ -437,8 +437,10 @@ double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8
                         if (grad_alpha) grad_alpha[b] += g * s * (off > 0 ? off : 0.0);
                         if (off > 0) {
                             if (grad_beta) grad_beta[b] -= g * s * alpha[b];
-                            const double qs = Qt[(e0 + src) * Z + r];
-                            gQn[(e0 + src) * Z + r] += g * s * alpha[b] * (qs < 0 ? -1.0 : 1.0);
+                            if (src >= 0) {
+                                const double qs = Qt[(e0 + src) * Z + r];
+                                gQn[(e0 + src) * Z + r] += g * s * alpha[b] * (qs < 0 ? -1.0 : 1.0);
+                            }
                         }
                     }
                 }

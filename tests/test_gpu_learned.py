@@ -1,0 +1,102 @@
+"""Learned offset/normalized min-sum on the GPU: forward parity, gradients, fallback."""
+import numpy as np
+import pytest
+
+from tests.oracle_binding import OR_BG1, OR_BG2, OR_WIFI
+from tests.test_gpu_decode import _gpu_decode, _noisy_llr
+
+pytestmark = pytest.mark.gpu
+KIND = {1: OR_BG1, 2: OR_BG2, 3: OR_WIFI}
+
+
+def _learned(torch, code, llr, n_iter, flags=0):
+    d_llr = torch.from_numpy(llr).cuda()
+    n = llr.shape[0]
+    d_hd = torch.zeros((n, code.hd_bytes), dtype=torch.uint8, device="cuda")
+    d_app = torch.zeros((n, code.n_vn), dtype=torch.float32, device="cuda")
+    d_it = torch.zeros(n, dtype=torch.int32, device="cuda")
+    code.decode_learned(d_llr.data_ptr(), n, n_iter, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(), flags=flags)
+    torch.cuda.synchronize()
+    hd = np.unpackbits(d_hd.cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
+    return hd, d_app.cpu().numpy(), d_it.cpu().numpy()
+
+
+@pytest.mark.parametrize("kind,Z", [(2, 208), (1, 384), (2, 52), (3, 96)])
+def test_plain_weights_equal_plain_decoder(nr, torch_gpu, kind, Z):
+    print(f"case learned-as-plain kind={kind} Z={Z}", flush=True)
+    code = nr.Code(kind, Z)
+    T = 8
+    code.set_weights(code.plain_weights(T))
+    llr, _ = _noisy_llr(nr, code, 6, 0.5, 0.25, seed=Z)
+    hd, app, _ = _learned(torch_gpu, code, llr, T)
+    phd, papp, _ = _gpu_decode(torch_gpu, code, llr, T)
+    assert np.array_equal(hd, phd)
+    assert np.array_equal(app.view(np.uint32), papp.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind,Z", [(2, 208), (2, 13), (1, 64), (3, 24)])
+def test_learned_matches_oracle(nr, oracle, torch_gpu, kind, Z):
+    print(f"case learned vs oracle kind={kind} Z={Z}", flush=True)
+    code = nr.Code(kind, Z)
+    T = 6
+    rng = np.random.default_rng(kind * 100 + Z)
+    w = np.stack([rng.uniform(0.5, 1.0, (T, code.nnz_full)), rng.uniform(-0.3, 0.5, (T, code.nnz_full))], -1)
+    w = w.astype(np.float32)
+    code.set_weights(w)
+    assert np.array_equal(code.get_weights(), w)
+    llr, _ = _noisy_llr(nr, code, 5, 1.0, 0.25, seed=Z + 1)
+    hd, app, _ = _learned(torch_gpu, code, llr, T)
+    oc = oracle.code(KIND[kind], Z)
+    ohd, oapp = oracle.decode_noms(oc, llr, T, w[..., 0].reshape(-1), w[..., 1].reshape(-1))
+    assert np.array_equal(hd, ohd)
+    assert np.array_equal(app, oapp)
+
+
+@pytest.mark.parametrize("kind,Z,T", [(2, 13, 4), (3, 8, 3), (1, 16, 3)])
+def test_gradient_matches_oracle(nr, oracle, torch_gpu, kind, Z, T):
+    """GPU backward vs the oracle's double-precision analytic gradient (itself checked
+    against finite differences on the CPU)."""
+    print(f"case gradient kind={kind} Z={Z} T={T}", flush=True)
+    torch = torch_gpu
+    code = nr.Code(kind, Z)
+    rng = np.random.default_rng(7 + Z)
+    w = np.stack([rng.uniform(0.6, 0.9, (T, code.nnz_full)), rng.uniform(-0.2, 0.2, (T, code.nnz_full))], -1)
+    w = w.astype(np.float32)
+    code.set_weights(w)
+    n_cw = 6
+    llr, bits = _noisy_llr(nr, code, n_cw, 1.0, 0.3, seed=Z)
+    d_llr = torch.from_numpy(llr).cuda()
+    d_bits = torch.from_numpy(bits).cuda()
+    grad = torch.zeros((T, code.nnz_full, 2), dtype=torch.float32, device="cuda")
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    ws = torch.empty(code.workspace_bytes(n_cw, T), dtype=torch.uint8, device="cuda")
+    code.train_grad(d_llr.data_ptr(), d_bits.data_ptr(), n_cw, T, grad.data_ptr(), loss.data_ptr(),
+                    ws.data_ptr(), ws.numel())
+    torch.cuda.synchronize()
+    oc = oracle.code(KIND[kind], Z)
+    oloss, ga, gb = oracle.noms_loss_grad(oc, llr, bits, T, w[..., 0].reshape(-1).astype(np.float64),
+                                          w[..., 1].reshape(-1).astype(np.float64))
+    g = grad.cpu().numpy()
+    assert abs(float(loss.item()) - oloss) <= 1e-4 * abs(oloss) + 1e-6
+    for got, want in ((g[..., 0].reshape(-1), ga), (g[..., 1].reshape(-1), gb)):
+        scale = np.abs(want).max()
+        assert scale > 0
+        assert np.abs(got - want).max() <= 2e-3 * scale, (np.abs(got - want).max(), scale)
+
+
+def test_fallback_to_plain(nr, torch_gpu):
+    """alpha = 0 freezes the learned decoder at the channel decision, so every noisy codeword
+    fails its syndrome and must come back as the plain decoder's output."""
+    print("case fallback BG2 Z=52", flush=True)
+    code = nr.Code(2, 52)
+    T = 10
+    w = code.plain_weights(T)
+    w[..., 0] = 0.0
+    code.set_weights(w)
+    llr, cw = _noisy_llr(nr, code, 32, 2.5, 0.2, seed=3)
+    lhd, _, _ = _learned(torch_gpu, code, llr, T)
+    fhd, fapp, _ = _learned(torch_gpu, code, llr, T, flags=nr.FALLBACK_PLAIN)
+    phd, papp, _ = _gpu_decode(torch_gpu, code, llr, T)
+    assert not np.array_equal(lhd, phd)
+    assert np.array_equal(fhd, phd)
+    assert np.array_equal(fapp, papp)
