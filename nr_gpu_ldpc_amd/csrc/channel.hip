@@ -86,6 +86,19 @@ __device__ __forceinline__ void nr_sincos_2pi(float u, float* sn, float* cs) {
     }
 }
 
+// Correctly rounded square root of a positive normal float: the hardware estimate is within
+// one ulp, so test its two neighbours with exact fma residuals and step to the right one.
+__device__ __forceinline__ float cr_sqrt(float x) {
+    float s = __builtin_amdgcn_sqrtf(x);
+    const uint32_t si = __float_as_uint(s);
+    const float dn = __uint_as_float(si - 1u), up = __uint_as_float(si + 1u);
+    const float rdn = fmaf(-dn, s, x);
+    const float rup = fmaf(-up, s, x);
+    s = (rdn <= 0.0f) ? dn : s;
+    s = (rup > 0.0f) ? up : s;
+    return s;
+}
+
 // Four standard normal samples 4*block .. 4*block+3 of a codeword's noise stream.
 __device__ __forceinline__ void gauss4(uint64_t seed, uint64_t stream, uint64_t block, float g[4]) {
     uint32_t o[4];
@@ -94,7 +107,7 @@ __device__ __forceinline__ void gauss4(uint64_t seed, uint64_t stream, uint64_t 
     for (int pr = 0; pr < 2; pr++) {
         const float u1 = (float)(o[2 * pr] >> 8) * 5.9604644775390625e-8f + 2.98023223876953125e-8f;
         const float u2 = (float)(o[2 * pr + 1] >> 8) * 5.9604644775390625e-8f;
-        const float rad = __fsqrt_rn(-2.0f * nr_log(u1));
+        const float rad = cr_sqrt(-2.0f * nr_log(u1));
         float sn, cs;
         nr_sincos_2pi(u2, &sn, &cs);
         g[2 * pr] = rad * cs;

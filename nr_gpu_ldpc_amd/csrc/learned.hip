@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(512) k_noms_backward(BackwardArgs a) {
     __syncthreads();
     for (int n = threadIdx.x; n < nacc; n += blockDim.x)
         if (gacc[n] != 0.f) atomicAdd(&a.grad[n], gacc[n]);
-    const float ls = wave_sum(act ? loss_part : 0.f);
+    const float ls = wave_sum(loss_part);  // every lane (r >= Z too) summed loss terms
     if ((threadIdx.x & 63) == 0) atomicAdd(a.loss, ls * a.inv_norm);
 }
 
