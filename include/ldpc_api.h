@@ -130,8 +130,10 @@ int ldpc_noms_get_weights(ldpc_ctx* ctx, int* n_iter, float* weights);
 
 /* Decode with the learned weights (n_iter must equal the weights' iteration count).  With
  * LDPC_FLAG_FALLBACK_PLAIN, codewords whose learned hard decision fails the syndrome check are
- * decoded again with plain min-sum (n_iter iterations); their outputs replace the learned ones.
- * Outputs as for ldpc_decode_async. */
+ * decoded again with plain min-sum (n_iter iterations); the plain outputs (hard decisions, APP,
+ * iterations) replace the learned ones for the codewords whose plain hard decision passes the
+ * syndrome check.  Outputs as for ldpc_decode_async; the fallback uses context scratch memory,
+ * so fallback calls on one context must not overlap. */
 int ldpc_decode_learned_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
                               unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 

@@ -158,6 +158,7 @@ struct CodeDesc {
     std::vector<int> V; // rows_full x cols_full, V mod Z or -1
 
     int n_vn() const { return cols * Z; }            // decoder columns incl. punctured ones
+    int hd_bytes() const { return (n_vn() + 7) / 8; } // packed hard-decision bytes
     int K() const { return kb * Z; }                 // information bits incl. punctured ones
     int N() const { return (cols - punct) * Z; }     // transmitted bits
     int M() const { return rows * Z; }

@@ -32,6 +32,21 @@ struct AdamArgs {
     float lr, b1, b2, eps, b1t, b2t;  // b1t = b1^step, b2t = b2^step
 };
 
+// Learned-decoder fallback: copy the plain re-decode of codeword cw into the outputs when the
+// learned result failed its syndrome check (w_learned != 0) and the plain one passes (w_plain == 0).
+struct MergeArgs {
+    const int* w_learned;
+    const int* w_plain;
+    const uint8_t* hd_plain;
+    const float* app_plain;  // NULL when no APP output is requested
+    const int* iters_plain;
+    uint8_t* hd;
+    float* app;
+    int* iters;              // may be NULL
+    int n_cw, hd_bytes, n_vn;
+};
+
+int fallback_merge_launch(const MergeArgs& a, hipStream_t s);
 int backward_launch(int kind, const BackwardArgs& a, int n_wg, hipStream_t s);
 int adam_launch(const AdamArgs& a, hipStream_t s);
 

@@ -177,6 +177,25 @@ __global__ void k_adam(AdamArgs a) {
     a.w[n] -= a.lr * mh / (sqrtf(vh) + a.eps);
 }
 
+// One workgroup per codeword; rows of codewords that keep their learned result are untouched.
+__global__ void __launch_bounds__(256) k_fallback_merge(MergeArgs a) {
+    const int cw = blockIdx.x;
+    if (a.w_learned[cw] == 0 || a.w_plain[cw] != 0) return;
+    const size_t hb = (size_t)cw * a.hd_bytes;
+    for (int n = threadIdx.x; n < a.hd_bytes; n += blockDim.x) a.hd[hb + n] = a.hd_plain[hb + n];
+    if (a.app) {
+        const size_t ab = (size_t)cw * a.n_vn;
+        for (int n = threadIdx.x; n < a.n_vn; n += blockDim.x) a.app[ab + n] = a.app_plain[ab + n];
+    }
+    if (a.iters && threadIdx.x == 0) a.iters[cw] = a.iters_plain[cw];
+}
+
+int fallback_merge_launch(const MergeArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_fallback_merge, dim3(a.n_cw), dim3(256), 0, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
 int backward_launch(int kind, const BackwardArgs& a, int n_wg, hipStream_t s) {
     const int threads = ((a.Z + 63) / 64) * 64;
     const size_t lds = (size_t)a.n_iter * a.nnz * 2 * sizeof(float);

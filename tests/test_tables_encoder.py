@@ -150,3 +150,16 @@ def test_noise_stddev_matches_oracle(nr, oracle, ebn0, rate):
     s = nr.noise_stddev(ebn0, rate)
     assert s == oracle.lib.oracle_noise_stddev(ebn0, rate)
     assert abs(s - np.sqrt(1 / rate / 10 ** (ebn0 / 10) / 2)) < 1e-6
+
+
+def test_committed_noms_weights_load(nr):
+    """The shipped learned weights have the base graph's edge count and sane values."""
+    import numpy as np
+    from nr_gpu_ldpc_amd import noms
+    rec = noms.load_weights(noms.weights_path(2, 208, 10))
+    info = nr.Code(2, 208, 0, -1)
+    assert (rec["kind"], rec["Z"], rec["n_iter"]) == (2, 208, 10)
+    assert rec["weights"].shape == (10, info.nnz_full, 2)
+    assert np.isfinite(rec["weights"]).all()
+    assert 0.2 < rec["weights"][..., 0].mean() < 1.5
+    assert noms.degree1_edges(2, 208).sum() == 38
