@@ -35,6 +35,7 @@ extern "C" {
 
 #define LDPC_FLAG_EARLY_EXIT 1u /* stop a codeword once its hard decision has zero syndrome */
 #define LDPC_FLAG_FALLBACK_PLAIN 2u /* learned decode: re-decode syndrome failures with plain min-sum */
+#define LDPC_FLAG_LEARNED 4u    /* mixed batches: learned weights where set, alpha = 0.75 elsewhere */
 
 typedef struct ldpc_ctx ldpc_ctx;
 
@@ -174,6 +175,72 @@ int ldpc_channel_async(ldpc_ctx* ctx, void* stream, uint64_t seed, uint64_t stre
  * (device int64).  Replaces cuda_error_check  src/encoding_tools.cpp:112-155. */
 int ldpc_count_errors_async(ldpc_ctx* ctx, void* stream, const uint8_t* hd_packed,
                             const uint8_t* cw_bits, int n_cw, long long* counts);
+
+/* ----------------------------------------------------------------- mixed batches
+ * Codewords of different base graphs, lifting sizes and rates decoded by ONE persistent launch
+ * with per-codeword early exit.  No reference counterpart: the reference decodes one code per
+ * process (the code lives in __constant__ memory, src/ldpc_kernels.cu:29-57).
+ *
+ * Usage: create once, plan a batch from host descriptors, then decode (directly, or through a
+ * captured HIP graph that is replayed for every later plan of the same object). */
+typedef struct ldpc_mixed ldpc_mixed;
+
+typedef struct {
+    int kind;              /* LDPC_CODE_BG1 or LDPC_CODE_BG2 */
+    int Z;                 /* lifting size (TS 38.212 set) */
+    int m_rows;            /* parity rows kept: 4..46 (BG1), 4..42 (BG2); rate kb/(kb + m_rows - 2) */
+    int reserved;          /* 0 */
+    long long llr_offset;  /* float index of the codeword's (kb + m_rows) * Z channel LLRs; app_out
+                              uses the same offsets */
+    long long hd_offset;   /* byte offset of its ceil((kb + m_rows) * Z / 8) packed hard decisions */
+} ldpc_mixed_cw;
+
+/* device < 0: host-only object (planning and plan inspection, no decoding).  max_cw bounds the
+ * codewords of one plan. */
+int ldpc_mixed_create(ldpc_mixed** out, int device, int max_cw);
+int ldpc_mixed_destroy(ldpc_mixed* p);
+
+/* Validate and plan a batch (host work: grouping by (kind, Z), tiling by codewords per
+ * workgroup, longest-first order).  Waits until the previous plan's upload has been consumed. */
+int ldpc_mixed_plan(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n_cw);
+
+/* Plan summary: tiles, launch threads, grid, LDS bytes; any pointer may be NULL. */
+int ldpc_mixed_plan_info(const ldpc_mixed* p, int* n_tiles, int* threads, int* grid, int* lds_bytes);
+/* Tiles of the current plan: out[t*6 + 0..5] = kind, Z, m_rows, cpb, first, count; the codeword
+ * order (descriptor index per planned slot) goes to order[n_cw] when non-NULL.  Returns the tile
+ * count (or a negative status). */
+int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order);
+
+/* Decode the planned batch: llr / hd / app_out are device buffers addressed by the descriptors'
+ * offsets; iters_out[i] (device, may be NULL) is the iteration count of descriptor i.
+ * flags: LDPC_FLAG_EARLY_EXIT, LDPC_FLAG_LEARNED. */
+int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags,
+                            uint8_t* hd, float* app_out, int* iters_out);
+
+/* Capture plan upload + decode (as ldpc_mixed_decode_async) into a HIP graph owned by p, with
+ * an optional host->device copy of llr_host (llr_floats floats into llr) before and a
+ * device->host copy of hd (hd_bytes bytes into hd_host) after.  Host buffers must be pinned
+ * (hipHostMalloc).  Replays decode whatever plan is current, into the same buffers. */
+int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
+                       float* app_out, int* iters_out, const float* llr_host, size_t llr_floats,
+                       uint8_t* hd_host, size_t hd_bytes);
+int ldpc_mixed_replay(ldpc_mixed* p, void* stream);
+
+/* Learned weights of one (kind, Z) for LDPC_FLAG_LEARNED decodes: host [n_iter][nnz_full][2]. */
+int ldpc_mixed_set_weights(ldpc_mixed* p, int kind, int Z, int n_iter, const float* weights);
+
+/* Codewords per workgroup (LDS codeword slots) of (kind, Z); 0 = default (the most that fit).
+ * ldpc_mixed_autotune times the candidate counts (each count that fills one more wavefront,
+ * 1 and the maximum) on n_cw full-rate codewords for n_iter iterations, keeps the fastest, and
+ * writes candidate k's count and microseconds per codeword to cand_cpb[k], us_per_cw[k] (k <
+ * max_candidates; either may be NULL).  It discards the current plan. */
+int ldpc_mixed_set_cpb(ldpc_mixed* p, int kind, int Z, int cpb);
+int ldpc_mixed_get_cpb(ldpc_mixed* p, int kind, int Z, int* cpb);
+int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, int* cand_cpb, float* us_per_cw,
+                        int max_candidates, int* n_candidates, int* best_cpb);
+/* Tuning cache: text lines "kind Z cpb". */
+int ldpc_mixed_save_tuning(const ldpc_mixed* p, const char* path);
+int ldpc_mixed_load_tuning(ldpc_mixed* p, const char* path);
 
 #ifdef __cplusplus
 }

@@ -10,8 +10,9 @@ torch tensor on ``cuda:0``) and streams as integer hipStream_t handles (0 = defa
 import ctypes
 import os
 
-__all__ = ["lib", "Code", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "setup_params",
-           "base_matrix", "noise_stddev", "generator_shifts", "device_count"]
+__all__ = ["lib", "Code", "Mixed", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "FALLBACK_PLAIN",
+           "LEARNED", "MIXED_CW_DTYPE", "setup_params", "base_matrix", "noise_stddev", "generator_shifts",
+           "device_count"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # NRLDPC_LIB selects another build of the same library (used to compare build variants).
@@ -20,6 +21,7 @@ LIB_PATH = os.environ.get("NRLDPC_LIB") or os.path.join(HERE, "lib", "libnrldpc.
 BG1, BG2, QC12X24 = 1, 2, 3
 EARLY_EXIT = 1
 FALLBACK_PLAIN = 2
+LEARNED = 4
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -91,6 +93,22 @@ _sig("ldpc_noms_workspace_bytes", ctypes.c_size_t, _P, ctypes.c_int, ctypes.c_in
 _sig("ldpc_noms_train_grad_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P,
      ctypes.c_size_t)
 _sig("ldpc_noms_adam_async", ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_float)
+_sig("ldpc_mixed_create", ctypes.c_int, ctypes.POINTER(_P), ctypes.c_int, ctypes.c_int)
+_sig("ldpc_mixed_destroy", ctypes.c_int, _P)
+_sig("ldpc_mixed_plan", ctypes.c_int, _P, _P, ctypes.c_int)
+_sig("ldpc_mixed_plan_info", ctypes.c_int, _P, _c_int_p, _c_int_p, _c_int_p, _c_int_p)
+_sig("ldpc_mixed_tiles", ctypes.c_int, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_mixed_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_mixed_capture", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P, _P, _P, ctypes.c_size_t,
+     _P, ctypes.c_size_t)
+_sig("ldpc_mixed_replay", ctypes.c_int, _P, _P)
+_sig("ldpc_mixed_set_weights", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P)
+_sig("ldpc_mixed_set_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int)
+_sig("ldpc_mixed_get_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _c_int_p)
+_sig("ldpc_mixed_autotune", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
+     ctypes.c_int, _c_int_p, _c_int_p)
+_sig("ldpc_mixed_save_tuning", ctypes.c_int, _P, ctypes.c_char_p)
+_sig("ldpc_mixed_load_tuning", ctypes.c_int, _P, ctypes.c_char_p)
 
 
 class LdpcError(RuntimeError):
@@ -240,3 +258,103 @@ class Code:
         _check(lib.ldpc_encode_host(self._h, info_bits.ctypes.data, info_bits.shape[0], out.ctypes.data),
                "ldpc_encode_host")
         return out
+
+
+def _mixed_cw_dtype():
+    import numpy as np
+    return np.dtype([("kind", "<i4"), ("Z", "<i4"), ("m_rows", "<i4"), ("reserved", "<i4"),
+                     ("llr_offset", "<i8"), ("hd_offset", "<i8")])
+
+
+try:
+    MIXED_CW_DTYPE = _mixed_cw_dtype()  # layout of ldpc_mixed_cw
+except ImportError:  # numpy absent: descriptors cannot be built from Python
+    MIXED_CW_DTYPE = None
+
+
+class Mixed:
+    """Persistent mixed-batch decoder (base graph, Z and parity rows per codeword).
+
+    Descriptors are a numpy array of MIXED_CW_DTYPE (kind, Z, m_rows, 0, llr_offset, hd_offset).
+    device < 0 makes a host-only planner (plan and tiles work, decoding does not)."""
+
+    def __init__(self, max_cw, device=0):
+        h = _P()
+        _check(lib.ldpc_mixed_create(ctypes.byref(h), device, int(max_cw)), "ldpc_mixed_create")
+        self._h = h
+        self.max_cw = max_cw
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ldpc_mixed_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, cws):
+        import numpy as np
+        self._cws = np.ascontiguousarray(cws, dtype=MIXED_CW_DTYPE)
+        _check(lib.ldpc_mixed_plan(self._h, self._cws.ctypes.data, len(self._cws)), "ldpc_mixed_plan")
+
+    def plan_info(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(lib.ldpc_mixed_plan_info(self._h, *[ctypes.byref(x) for x in v]), "ldpc_mixed_plan_info")
+        return dict(zip(("n_tiles", "threads", "grid", "lds_bytes"), (x.value for x in v)))
+
+    def tiles(self):
+        """(tiles [n_tiles, 6] = kind, Z, m_rows, cpb, first, count; order [n_cw])."""
+        import numpy as np
+        n = self.plan_info()["n_tiles"]
+        out = np.zeros((max(n, 1), 6), dtype=np.int32)
+        order = np.zeros(max(len(self._cws), 1), dtype=np.int32)
+        r = lib.ldpc_mixed_tiles(self._h, out.ctypes.data, n, order.ctypes.data)
+        if r < 0:
+            _check(r, "ldpc_mixed_tiles")
+        return out[:n], order[:len(self._cws)]
+
+    def decode(self, llr_ptr, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        _check(lib.ldpc_mixed_decode_async(self._h, stream, llr_ptr, n_iter, flags, hd_ptr, app_ptr or None,
+                                           iters_ptr or None), "ldpc_mixed_decode_async")
+
+    def capture(self, stream, llr_ptr, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, llr_host_ptr=0,
+                llr_floats=0, hd_host_ptr=0, hd_bytes=0):
+        _check(lib.ldpc_mixed_capture(self._h, stream, llr_ptr, n_iter, flags, hd_ptr, app_ptr or None,
+                                      iters_ptr or None, llr_host_ptr or None, llr_floats, hd_host_ptr or None,
+                                      hd_bytes), "ldpc_mixed_capture")
+
+    def replay(self, stream):
+        _check(lib.ldpc_mixed_replay(self._h, stream), "ldpc_mixed_replay")
+
+    def set_weights(self, kind, Z, weights):
+        import numpy as np
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        _check(lib.ldpc_mixed_set_weights(self._h, kind, Z, w.shape[0], w.ctypes.data), "ldpc_mixed_set_weights")
+
+    def set_cpb(self, kind, Z, cpb):
+        _check(lib.ldpc_mixed_set_cpb(self._h, kind, Z, cpb), "ldpc_mixed_set_cpb")
+
+    def get_cpb(self, kind, Z):
+        c = ctypes.c_int()
+        _check(lib.ldpc_mixed_get_cpb(self._h, kind, Z, ctypes.byref(c)), "ldpc_mixed_get_cpb")
+        return c.value
+
+    def autotune(self, kind, Z, n_iter, n_cw):
+        """Times every candidate codewords-per-workgroup count; returns (best, {cpb: us_per_cw})."""
+        import numpy as np
+        cand = np.zeros(16, dtype=np.int32)
+        us = np.zeros(16, dtype=np.float32)
+        n, best = ctypes.c_int(), ctypes.c_int()
+        _check(lib.ldpc_mixed_autotune(self._h, kind, Z, n_iter, n_cw, cand.ctypes.data, us.ctypes.data, 16,
+                                       ctypes.byref(n), ctypes.byref(best)), "ldpc_mixed_autotune")
+        k = min(n.value, 16)
+        return best.value, {int(c): float(u) for c, u in zip(cand[:k], us[:k])}
+
+    def save_tuning(self, path):
+        _check(lib.ldpc_mixed_save_tuning(self._h, os.fsencode(path)), "ldpc_mixed_save_tuning")
+
+    def load_tuning(self, path):
+        _check(lib.ldpc_mixed_load_tuning(self._h, os.fsencode(path)), "ldpc_mixed_load_tuning")

@@ -37,7 +37,7 @@ COMMON = [
 ] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
 SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
-           "encoder.cpp", "noms_api.cpp", "compat.cpp", "driver.cpp", "autotune.cpp", "mixed.hip"]
+           "encoder.cpp", "noms_api.cpp", "compat.cpp", "driver.cpp", "mixed_plan.cpp", "mixed.hip"]
 
 
 def _compile(src):

@@ -66,13 +66,13 @@ static int upload(std::vector<T> const& h, T** d) {
     return LDPC_OK;
 }
 
-int ctx_build_device(ldpc_ctx* x) {
-    const CodeDesc& c = x->code;
+void build_edge_params(const CodeDesc& c, std::vector<EdgeParam>* out) {
     const int Z = c.Z;
     std::vector<int> er, ec;
     edge_list(c, &er, &ec);
     const int NND = n_nd(c.kind);
-    std::vector<EdgeParam> et(er.size());
+    std::vector<EdgeParam>& et = *out;
+    et.assign(er.size(), EdgeParam{0u, 0u});
     for (size_t n = 0; n < er.size(); n++) {
         const int v = c.shift(er[n], ec[n]);
         const int j = ec[n];
@@ -84,6 +84,14 @@ int ctx_build_device(ldpc_ctx* x) {
             et[n].acc = 0;
         }
     }
+}
+
+int ctx_build_device(ldpc_ctx* x) {
+    const CodeDesc& c = x->code;
+    std::vector<int> er, ec;
+    edge_list(c, &er, &ec);
+    std::vector<EdgeParam> et;
+    build_edge_params(c, &et);
     int st = upload(et, &x->d_etab);
     if (st) return st;
     // syndrome tables over the rows in use

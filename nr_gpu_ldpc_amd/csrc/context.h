@@ -39,6 +39,8 @@ namespace nrldpc {
 int nnz_full(int kind);
 int n_nd(int kind);
 int default_cpb(const CodeDesc& c);
+// Per-edge LDS offsets of the any-Z decoder for code c (row-major edge order, whole base graph).
+void build_edge_params(const CodeDesc& c, std::vector<EdgeParam>* out);
 // Shared decode path; weights != null selects the learned decoder with [(n_iter+1)][nnz] weights.
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* select);

@@ -39,6 +39,46 @@ struct DecodeArgs {
     const int* select;     // [n_cw] or null: decode only codewords with select[cw] != 0
 };
 
+// One codeword of a mixed batch (device table, built by the planner).
+struct MixedCw {
+    long long llr_off;  // float offset of the codeword's LLR row (also its APP row)
+    long long hd_off;   // byte offset of its packed hard decisions
+    int id;             // position in the caller's descriptor array (iteration output)
+    int m_rows;         // parity rows kept
+    int n_vn;           // (kb + m_rows) * Z
+    int hd_bytes;       // ceil(n_vn / 8)
+};
+
+// One work item of the persistent mixed-batch kernel: up to cpb codewords of one (kind, Z).
+struct MixedTile {
+    const EdgeParam* etab;  // per-edge table of (kind, Z)
+    const float2* weights;  // learned weights of (kind, Z) or null
+    int kind, Z;
+    int m_rows;             // largest parity-row count among the tile's codewords
+    int cpb;                // LDS codeword slots (lanes used: cpb * Z)
+    int first, count;       // codewords first .. first + count - 1 of the MixedCw list
+    int lds_cw_bytes;
+    int pad;
+};
+
+struct MixedArgs {
+    const MixedTile* tiles;
+    const MixedCw* cws;
+    unsigned* queue;        // [0] next tile, [1] finished workgroups (both zero between launches)
+    const float* llr;
+    uint8_t* hd;
+    float* app_out;         // or null
+    int* iters_out;         // [n descriptors] or null
+    const int* n_tiles;     // device word: tiles in the list (read by the kernel, so a captured
+                            // launch serves any plan that fits the buffers)
+    int n_iter, early_exit;
+    int slot_off;           // LDS byte offset of the work-queue broadcast word
+};
+
+int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);
+// Resident workgroups per CU of the mixed kernel at this shape (0 on error).
+int mixed_occupancy(bool learned, int threads, size_t lds);
+
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
 int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);

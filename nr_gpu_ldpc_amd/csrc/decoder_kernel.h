@@ -143,26 +143,51 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
     }
 }
 
-// ZC > 0: kernel specialised for one lifting size, every circulant offset is an immediate.
+// One workgroup's share of a decode: codewords blk*cpb .. blk*cpb+cpb-1.
+// ZC > 0: specialised for one lifting size, every circulant offset is an immediate.
 // ZC == 0: any lifting size, offsets come from the per-edge table (scalar loads).
 // TRAIN (learned only): also store, per iteration and expanded row, the two minima, the
 // first-strict-minimum and second-minimum edge indices and the Q sign bits, for the
 // backward pass (layout [cw][iteration][row][Z]).
-template <int KIND, int ZC, bool LEARNED, bool TRAIN = false>
-__global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
+// MIXED (persistent mixed-batch kernel, ZC == 0): the workgroup decodes one tile of a mixed
+// batch.  `mcw` lists the tile's codewords; each has its own parity-row count (at most
+// a.m_rows, the tile's maximum), length and output location.  Rows beyond a codeword's own
+// count are computed with zero channel LLRs on their extension columns and their VN updates
+// are forced to +0.0, which leaves every APP bit pattern unchanged (APP is never -0.0).
+// Lanes at or above cpb*Z shadow lane (cpb-1, tid mod Z): they compute the same values and
+// skip the one non-idempotent store (the accumulator read-modify-write) and the outputs.
+template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED>
+__device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
+                                             const MixedCw* __restrict__ mcw) {
     using S = Structure<KIND>;
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
+    static_assert(!(MIXED && (TRAIN || ZC > 0)), "mixed tiles use the any-Z plain/learned path");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
 
     const int Z = ZC > 0 ? ZC : a.Z;
     const int tid = threadIdx.x;
-    const int cwl = tid / Z;
+    int cwl = tid / Z;
     const int r = tid - cwl * Z;
-    const int cw = blockIdx.x * a.cpb + cwl;
-    const bool live = cw < a.n_cw && (a.select == nullptr || a.select[cw] != 0);
-    const int cwc = live ? cw : a.n_cw - 1;
+    bool dead = false;  // MIXED: lane beyond the tile's cpb*Z lanes
+    if constexpr (MIXED) {
+        if (cwl >= a.cpb) { dead = true; cwl = a.cpb - 1; }
+    }
+    const int cw = blk * a.cpb + cwl;
+    bool live;
+    int cwc;
+    if constexpr (MIXED) {
+        live = !dead && cwl < a.n_cw;
+        cwc = cwl < a.n_cw ? cwl : a.n_cw - 1;  // index into mcw
+    } else {
+        live = cw < a.n_cw && (a.select == nullptr || a.select[cw] != 0);
+        cwc = live ? cw : a.n_cw - 1;
+    }
+    int m_lane = a.m_rows;  // parity rows of this lane's codeword
+    if constexpr (MIXED) m_lane = mcw[cwc].m_rows;
+    // MIXED: a wavefront whose lanes are all beyond cpb*Z only takes part in the barriers.
+    const bool idle = MIXED && (int)(__builtin_amdgcn_readfirstlane(tid) & ~63) >= a.cpb * Z;
     const uint32_t Z4 = 4u * Z;
     const uint32_t base = lds_base() + cwl * a.lds_cw_bytes;
     uint32_t rRd = base + 4u * r;                  // read copy, block column 0
@@ -186,15 +211,17 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     for (int i = 0; i < NW; i++) W[i] = 0u;
 
     // ---- load channel LLRs
-    const float* __restrict__ L = a.llr + (size_t)cwc * a.n_vn;
+    const float* __restrict__ L = MIXED ? a.llr + mcw[cwc].llr_off : a.llr + (size_t)cwc * a.n_vn;
+    if (!idle) {
 #pragma unroll 2
-    for (int j = 0; j < NND; j++) {
-        const float v = L[j * Z + r] + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
-        lds_st(rRd + j * 2 * Z4, v);
-        lds_st(rRd + j * 2 * Z4 + Z4, v);
-        lds_st(rAcc + j * Z4, v);
+        for (int j = 0; j < NND; j++) {
+            const float v = L[j * Z + r] + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
+            lds_st(rRd + j * 2 * Z4, v);
+            lds_st(rRd + j * 2 * Z4 + Z4, v);
+            lds_st(rAcc + j * Z4, v);
+        }
     }
-    static_for<NND, S::COLS>([&](auto JC) {
+    if (!idle) static_for<NND, S::COLS>([&](auto JC) {
         constexpr int j = JC;
         constexpr int e = S::T.d1_edge[j];
         constexpr int i = S::T.d1_row[j];
@@ -203,7 +230,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             if constexpr (ZC > 0) c = r + edge_vm<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            app1[S::T.d1_slot[j]] = L[j * Z + c] + 0.0f;
+            if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = L[j * Z + c] + 0.0f;
+            else app1[S::T.d1_slot[j]] = 0.0f;
         }
     });
     __syncthreads();
@@ -221,7 +249,14 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             float* ap_out = a.app_out;
             CEdge* ee = et;
             asm volatile("" : "+v"(eb), "+v"(er), "+s"(ap_out), "+s"(ee));
-            if (ap_out) ap_out += (size_t)cw * a.n_vn;
+            int ml = m_rows, nv = a.n_vn;  // this codeword's parity rows and length
+            if constexpr (MIXED) {
+                if (ap_out) ap_out += mcw[cwc].llr_off;
+                ml = mcw[cwc].m_rows;
+                nv = mcw[cwc].n_vn;
+            } else if (ap_out) {
+                ap_out += (size_t)cw * a.n_vn;
+            }
 #pragma unroll 1
             for (int j = 0; j < NND; j++) {
                 const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
@@ -232,7 +267,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                 constexpr int j = JC;
                 constexpr int e = S::T.d1_edge[j];
                 constexpr int i = S::T.d1_row[j];
-                if (i < m_rows) {
+                if (i < ml) {
                     int c;
                     if constexpr (ZC > 0) c = er + edge_vm<KIND, ZC>(e);
                     else c = er + (int)ee[e].off_rd;
@@ -242,17 +277,25 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                     if (ap_out) ap_out[j * Z + c] = x;
                 }
             });
-            if (er < 8) lds_stb(eb + a.n_vn + er, 0);
+            if (er < 8) lds_stb(eb + nv + er, 0);
         }
         __syncthreads();
         if (mine) {
-            for (int b = r; b < a.hd_bytes; b += Z) {
+            int hb = a.hd_bytes;
+            uint8_t* hrow = a.hd + (size_t)cw * a.hd_bytes;
+            int* itp = a.iters_out ? a.iters_out + cw : nullptr;
+            if constexpr (MIXED) {
+                hb = mcw[cwc].hd_bytes;
+                hrow = a.hd + mcw[cwc].hd_off;
+                itp = a.iters_out ? a.iters_out + mcw[cwc].id : nullptr;
+            }
+            for (int b = r; b < hb; b += Z) {
                 const uint32_t v0 = lds_ldu(base + 8u * b), v1 = lds_ldu(base + 8u * b + 4u);
                 const uint32_t lo = (v0 | (v0 >> 7) | (v0 >> 14) | (v0 >> 21)) & 0xFu;
                 const uint32_t hi = (v1 | (v1 >> 7) | (v1 >> 14) | (v1 >> 21)) & 0xFu;
-                a.hd[(size_t)cw * a.hd_bytes + b] = (uint8_t)(lo | (hi << 4));
+                hrow[b] = (uint8_t)(lo | (hi << 4));
             }
-            if (r == 0 && a.iters_out) a.iters_out[cw] = iters;
+            if (r == 0 && itp) *itp = iters;
         }
         __syncthreads();
     };
@@ -280,88 +323,93 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
             int rr = r;
             uint32_t rrHi = rHi;
             asm volatile("" : "+v"(rr), "+v"(rrHi));
-            float q[D], ro[D];
-            float m1 = 1000.0f, m2 = 1000.0f;
-            uint32_t neg = 0u;
-            const float a1 = A1[i], a2 = A2[i];
-            const uint32_t wo = W[WW];
-            uint32_t a1i = 31u, a2i = 31u;  // TRAIN: arg-min / second arg-min (31 = none)
-            uint32_t idxo = (wo >> IO) & 31u;
-            asm volatile("" : "+v"(idxo));
-            // pass 1: Q = APP_old - R_old, two-min, signs
-            static_for<0, D>([&](auto KC) {
-                constexpr int k = KC;
-                constexpr int e = E0 + k;
-                constexpr int j = S::T.ecol[e];
-                float x;
-                if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
-                else x = app1[S::T.d1_slot[j]];
-                const float mo = (idxo == (uint32_t)k) ? a2 : a1;
-                float rv;
-                if constexpr (LEARNED) {
-                    const float2 w = w_old[e];
-                    const float mag = w.x * fmaxf(mo - w.y, 0.0f);
-                    rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
-                } else {
-                    rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
-                }
-                const float qv = x - rv;
-                q[k] = qv;
-                ro[k] = rv;
-                const float aq = fabsf(qv);
-                if constexpr (TRAIN) {
-                    const bool lt1 = aq < m1, lt2 = aq < m2;
-                    a2i = lt1 ? a1i : (lt2 ? (uint32_t)k : a2i);
-                    a1i = lt1 ? (uint32_t)k : a1i;
-                }
-                m2 = __builtin_amdgcn_fmed3f(m1, m2, aq);
-                m1 = fminf(m1, aq);
-                // sign bit of Q; edge k lands at bit D-1-k.  APP is never -0.0 (the channel
-                // LLRs are canonicalised on load and x + y = -0.0 needs x = y = -0.0), so Q is
-                // never -0.0 and the raw sign bit equals the reference's (Q < 0).
-                neg = __builtin_amdgcn_alignbit(neg, as_u(qv), 31);
-            });
-            const uint32_t sgn = __popc(neg) & 1u;
-            const uint32_t sbit = sgn << 31;
-            if constexpr (TRAIN) {
-                if (live) {
-                    const size_t ri = (((size_t)cw * a.n_iter + t) * ROWS + i) * (size_t)Z + r;
-                    a.rec_m1[ri] = m1;
-                    a.rec_m2[ri] = m2;
-                    a.rec_w[ri] = neg | (a1i << 20) | (a2i << 25);
-                }
-            }
-            float n1, n2;
-            if constexpr (LEARNED) { n1 = m1; n2 = m2; }
-            else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
-            const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
-            // pass 2: R_new = (product of the other signs) * scaled min, dt = R_new - R_old
-            uint32_t idxn = 0u;
             float dtv[D];
-            static_for<0, D>([&](auto KC) {
-                constexpr int k = KC;
-                constexpr int e = E0 + k;
-                const bool eq = fabsf(q[k]) == m1;
-                idxn = eq ? (uint32_t)k : idxn;
-                float ms;
-                if constexpr (LEARNED) {
-                    const float2 w = w_new[e];
-                    const float mag = w.x * fmaxf((eq ? n2 : n1) - w.y, 0.0f);
-                    ms = as_f(as_u(mag) ^ sbit);
-                } else {
-                    ms = eq ? n2s : n1s;
+            if (!idle) {
+                float q[D], ro[D];
+                float m1 = 1000.0f, m2 = 1000.0f;
+                uint32_t neg = 0u;
+                const float a1 = A1[i], a2 = A2[i];
+                const uint32_t wo = W[WW];
+                uint32_t a1i = 31u, a2i = 31u;  // TRAIN: arg-min / second arg-min (31 = none)
+                uint32_t idxo = (wo >> IO) & 31u;
+                asm volatile("" : "+v"(idxo));
+                // pass 1: Q = APP_old - R_old, two-min, signs
+                static_for<0, D>([&](auto KC) {
+                    constexpr int k = KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    float x;
+                    if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    else x = app1[S::T.d1_slot[j]];
+                    const float mo = (idxo == (uint32_t)k) ? a2 : a1;
+                    float rv;
+                    if constexpr (LEARNED) {
+                        const float2 w = w_old[e];
+                        const float mag = w.x * fmaxf(mo - w.y, 0.0f);
+                        rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
+                    } else {
+                        rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
+                    }
+                    const float qv = x - rv;
+                    q[k] = qv;
+                    ro[k] = rv;
+                    const float aq = fabsf(qv);
+                    if constexpr (TRAIN) {
+                        const bool lt1 = aq < m1, lt2 = aq < m2;
+                        a2i = lt1 ? a1i : (lt2 ? (uint32_t)k : a2i);
+                        a1i = lt1 ? (uint32_t)k : a1i;
+                    }
+                    m2 = __builtin_amdgcn_fmed3f(m1, m2, aq);
+                    m1 = fminf(m1, aq);
+                    // sign bit of Q; edge k lands at bit D-1-k.  APP is never -0.0 (the channel
+                    // LLRs are canonicalised on load and x + y = -0.0 needs x = y = -0.0), so Q is
+                    // never -0.0 and the raw sign bit equals the reference's (Q < 0).
+                    neg = __builtin_amdgcn_alignbit(neg, as_u(qv), 31);
+                });
+                const uint32_t sgn = __popc(neg) & 1u;
+                const uint32_t sbit = sgn << 31;
+                if constexpr (TRAIN) {
+                    if (live) {
+                        const size_t ri = (((size_t)cw * a.n_iter + t) * ROWS + i) * (size_t)Z + r;
+                        a.rec_m1[ri] = m1;
+                        a.rec_m2[ri] = m2;
+                        a.rec_w[ri] = neg | (a1i << 20) | (a2i << 25);
+                    }
                 }
-                const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
-                dtv[k] = rn - ro[k];
-            });
-            A1[i] = n1;
-            A2[i] = n2;
-            constexpr uint32_t dmask = (1u << D) - 1u;
-            const uint32_t fresh_w = ((sgn ? ~neg : neg) & dmask) | (idxn << (IO - WO));
-            if constexpr (!S::half(i)) W[WW] = fresh_w;
-            else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
-            else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
+                float n1, n2;
+                if constexpr (LEARNED) { n1 = m1; n2 = m2; }
+                else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
+                const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
+                const bool row_on = !MIXED || i < m_lane;  // MIXED: row kept by this codeword
+                // pass 2: R_new = (product of the other signs) * scaled min, dt = R_new - R_old
+                uint32_t idxn = 0u;
+                static_for<0, D>([&](auto KC) {
+                    constexpr int k = KC;
+                    constexpr int e = E0 + k;
+                    const bool eq = fabsf(q[k]) == m1;
+                    idxn = eq ? (uint32_t)k : idxn;
+                    float ms;
+                    if constexpr (LEARNED) {
+                        const float2 w = w_new[e];
+                        const float mag = w.x * fmaxf((eq ? n2 : n1) - w.y, 0.0f);
+                        ms = as_f(as_u(mag) ^ sbit);
+                    } else {
+                        ms = eq ? n2s : n1s;
+                    }
+                    const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
+                    if constexpr (MIXED) dtv[k] = row_on ? rn - ro[k] : 0.0f;
+                    else dtv[k] = rn - ro[k];
+                });
+                A1[i] = n1;
+                A2[i] = n2;
+                constexpr uint32_t dmask = (1u << D) - 1u;
+                const uint32_t fresh_w = ((sgn ? ~neg : neg) & dmask) | (idxn << (IO - WO));
+                if constexpr (!S::half(i)) W[WW] = fresh_w;
+                else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
+                else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
+            }  // !idle
             if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
+            if (idle) return;
             // VN update, ascending block row per column
 #if NRLDPC_LDS_ATOMIC_ACC
             static_for<0, D>([&](auto KC) {
@@ -370,6 +418,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                 constexpr int j = S::T.ecol[e];
                 if constexpr (j < NND) {
                     const uint32_t addr = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                    if (!MIXED || !dead)
                     __hip_atomic_fetch_add((lds_f32*)(uintptr_t)addr, dtv[k], __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_WORKGROUP);
                 } else {
@@ -398,7 +447,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                     constexpr int e = E0 + k;
                     constexpr int j = S::T.ecol[e];
                     if constexpr (j < NND) {
-                        lds_st(addr[KC], cur[KC] + dtv[k]);
+                        if (!MIXED || !dead) lds_st(addr[KC], cur[KC] + dtv[k]);
                     } else {
                         float& ap = app1[S::T.d1_slot[j]];
                         ap = ap + dtv[k];
@@ -410,18 +459,20 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
         __syncthreads();
         if (t + 1 == a.n_iter) break;
         // APP_new becomes the read copy of the next iteration.
+        if (!idle) {
 #pragma unroll 2
-        for (int j = 0; j < NND; j++) {
-            const float v = lds_ld(rAcc + j * Z4);
-            lds_st(rRd + j * 2 * Z4, v);
-            lds_st(rRd + j * 2 * Z4 + Z4, v);
+            for (int j = 0; j < NND; j++) {
+                const float v = lds_ld(rAcc + j * Z4);
+                lds_st(rRd + j * 2 * Z4, v);
+                lds_st(rRd + j * 2 * Z4 + Z4, v);
+            }
         }
-        if (early && r == 0) lds_stu(flagAddr, 0u);
+        if (early && r == 0 && !idle) lds_stu(flagAddr, 0u);
         __syncthreads();
         if (early) {
             // syndrome of the current hard decision; any unsatisfied check keeps decoding
             uint32_t bad = 0u;
-            static_for<0, ROWS>([&](auto IC) {
+            if (!idle) static_for<0, ROWS>([&](auto IC) {
                 constexpr int i = IC;
                 constexpr int D = S::deg(i);
                 constexpr int E0 = S::T.row_start[i];
@@ -438,7 +489,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
                     else x = app1[S::T.d1_slot[j]];
                     p ^= x > 0.f ? 0u : 1u;
                 });
-                bad |= p;
+                if constexpr (MIXED) bad |= (i < m_lane) ? p : 0u;
+                else bad |= p;
             });
             if (bad) lds_stu(flagAddr, 1u);
             __syncthreads();
@@ -454,6 +506,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     }
     if (block_any(live && !emitted, wgFlag))
         emit(live && !emitted, t < a.n_iter ? t + 1 : a.n_iter);
+}
+
+template <int KIND, int ZC, bool LEARNED, bool TRAIN = false>
+__global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
+    decode_block<KIND, ZC, LEARNED, TRAIN, false>(a, blockIdx.x, nullptr);
 }
 
 }  // namespace nrldpc

@@ -1,0 +1,75 @@
+// mixed.hip -- persistent decoder for batches that mix base graphs, lifting sizes and rates.
+//
+// The host planner (mixed_plan.cpp) groups the batch into tiles of one (base graph, Z) each, at
+// most cpb codewords per tile, ordered longest first.  A grid sized to the resident
+// workgroups pulls tiles from a device work counter until the list is exhausted; every tile
+// runs the any-Z fused decoder body with per-codeword row counts, early exit and outputs.
+// The last workgroup to finish clears the counters, so consecutive launches (and graph
+// replays) need no reset.
+#include "status.h"
+#include "decoder_kernel.h"
+
+namespace nrldpc {
+
+template <bool LEARNED>
+__global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
+    const uint32_t slot = lds_base() + (uint32_t)m.slot_off;
+    const int n_tiles = *m.n_tiles;
+    for (;;) {
+        __syncthreads();  // the previous tile is done with the broadcast word
+        if (threadIdx.x == 0) lds_stu(slot, atomicAdd(&m.queue[0], 1u));
+        __syncthreads();
+        const int ti = __builtin_amdgcn_readfirstlane((int)lds_ldu(slot));
+        if (ti >= n_tiles) break;
+        // Tile fields are wave-uniform: read them through the constant address space so that
+        // they land in SGPRs (scalar loads).
+        using CTile = const MixedTile __attribute__((address_space(4)));
+        CTile* t = (CTile*)(m.tiles) + ti;
+        const int kind = t->kind;
+        DecodeArgs a{};
+        a.llr = m.llr;
+        a.hd = m.hd;
+        a.app_out = m.app_out;
+        a.iters_out = m.iters_out;
+        a.etab = t->etab;
+        a.weights = t->weights;
+        a.n_cw = t->count;
+        a.Z = t->Z;
+        a.m_rows = t->m_rows;
+        a.n_iter = m.n_iter;
+        a.cpb = t->cpb;
+        a.early_exit = m.early_exit;
+        a.lds_cw_bytes = t->lds_cw_bytes;
+        a.nnz = kind == kBG1 ? Structure<kBG1>::NNZ : Structure<kBG2>::NNZ;
+        const MixedCw* cws = m.cws + t->first;
+        if (kind == kBG1) decode_block<kBG1, 0, LEARNED, false, true>(a, 0, cws);
+        else decode_block<kBG2, 0, LEARNED, false, true>(a, 0, cws);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&m.queue[1], 1u) == gridDim.x - 1) {  // every other workgroup has left
+            atomicExch(&m.queue[0], 0u);
+            atomicExch(&m.queue[1], 0u);
+        }
+    }
+}
+
+int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s) {
+    if (threads > kMaxBlock || threads < 64 || lds > (size_t)max_lds_bytes() || grid < 1) {
+        set_last_error("mixed launch exceeds the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
+    if (learned) hipLaunchKernelGGL(k_decode_mixed<true>, dim3(grid), dim3(threads), lds, s, a);
+    else hipLaunchKernelGGL(k_decode_mixed<false>, dim3(grid), dim3(threads), lds, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
+int mixed_occupancy(bool learned, int threads, size_t lds) {
+    int n = 0;
+    hipError_t e = learned ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<true>, threads, lds)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<false>, threads, lds);
+    return e == hipSuccess ? n : 0;
+}
+
+}  // namespace nrldpc

@@ -1,0 +1,559 @@
+// mixed_plan.cpp -- planner, autotuner, graph capture and C ABI of the persistent mixed-batch
+// decoder (kernel in mixed.hip).
+//
+// Plan of a batch (host):
+//   1. group the codewords by (base graph, Z) and sort each group by parity rows, largest first;
+//   2. cut every group into tiles of cpb codewords (cpb per (base graph, Z): tuned or default);
+//      a tile runs with the largest row count of its codewords, the others mask the extra rows;
+//   3. order the tiles longest first (estimated by SIMD-rounds x edges), so the work queue of
+//      the persistent grid ends with the short tiles.
+// The plan lives in pinned staging memory and is copied to the device on the decode stream;
+// the kernel reads the tile count from the device copy, so one captured graph serves every
+// later plan of the same object.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ldpc_api.h"
+#include "code.h"
+#include "context.h"
+#include "decoder.h"
+#include "status.h"
+
+namespace nrldpc {
+
+constexpr int kMixedThreads = 512;  // workgroup size of every mixed launch
+constexpr int kPlainRows = 65;      // rows of the alpha = 0.75 weight tables (n_iter <= 64)
+constexpr size_t kHdrBytes = 256;
+
+struct MixedCode {
+    CodeDesc code;                 // whole base graph
+    EdgeParam* d_etab = nullptr;
+    int lds_cw = 0;                // LDS bytes per codeword slot
+    int cpb_max = 1, cpb_default = 1, cpb = 0;  // cpb 0: default
+    float2* d_weights = nullptr;   // trained weights [(T+1)][nnz_full], row 0 zero
+    int weights_iters = 0;
+};
+
+static int code_key(int kind, int Z) { return kind * 1024 + Z; }
+
+static int edges_in_rows(int kind, int m) {
+    int n = 0;
+    if (kind == kBG1) { for (const auto& e : kBG1Entries) n += e.row < m; }
+    else { for (const auto& e : kBG2Entries) n += e.row < m; }
+    return n;
+}
+
+}  // namespace nrldpc
+
+using namespace nrldpc;
+
+struct ldpc_mixed {
+    int device = -1;
+    int max_cw = 0;
+    int n_cu = 256, grid = 256, threads = kMixedThreads;
+    int lds = 0;                       // dynamic LDS bytes per workgroup
+    std::map<int, MixedCode> codes;
+    unsigned char* h_plan = nullptr;   // pinned staging: [header][tiles][codewords]
+    unsigned char* d_plan = nullptr;
+    size_t tiles_off = kHdrBytes, cws_off = 0, plan_bytes = 0;
+    unsigned* d_queue = nullptr;
+    float2* d_plain_w[2] = {nullptr, nullptr};
+    int n_tiles = 0, n_cw = 0;
+    std::vector<int> order, tiles6;
+    int plan_trained_T = 0;
+    bool planned = false;
+    hipEvent_t uploaded = nullptr;
+    bool upload_pending = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::mutex mu;
+};
+
+namespace nrldpc {
+
+static int mixed_get_code(ldpc_mixed* p, int kind, int Z, MixedCode** out) {
+    auto it = p->codes.find(code_key(kind, Z));
+    if (it != p->codes.end()) { *out = &it->second; return LDPC_OK; }
+    if (kind != kBG1 && kind != kBG2) {
+        set_last_error("mixed batches take base graph 1 or 2");
+        return LDPC_EINVAL;
+    }
+    MixedCode mc;
+    std::string err;
+    int st = make_code(kind, Z, 0, &mc.code, &err);
+    if (st) { set_last_error(err); return st; }
+    mc.lds_cw = (int)decoder_lds_bytes_per_cw(kind, Z);
+    const int slots = p->lds - 32;  // workgroup flag and queue word above the codeword slots
+    mc.cpb_max = std::max(1, std::min(p->threads / Z, slots / mc.lds_cw));
+    if (mc.lds_cw > slots || Z > p->threads) {
+        set_last_error("lifting size does not fit one workgroup");
+        return LDPC_EINVAL;
+    }
+    mc.cpb_default = mc.cpb_max;  // the most lanes in use (the autotuner may choose fewer)
+    if (p->device >= 0) {
+        std::vector<EdgeParam> et;
+        build_edge_params(mc.code, &et);
+        NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&mc.d_etab), et.size() * sizeof(EdgeParam)));
+        NRLDPC_HIP(hipMemcpy(mc.d_etab, et.data(), et.size() * sizeof(EdgeParam), hipMemcpyHostToDevice));
+    }
+    auto res = p->codes.emplace(code_key(kind, Z), mc);
+    *out = &res.first->second;
+    return LDPC_OK;
+}
+
+static int mixed_cpb(const MixedCode& mc) {
+    return mc.cpb > 0 ? std::min(mc.cpb, mc.cpb_max) : mc.cpb_default;
+}
+
+static int wait_upload(ldpc_mixed* p) {
+    if (p->upload_pending) {
+        NRLDPC_HIP(hipEventSynchronize(p->uploaded));
+        p->upload_pending = false;
+    }
+    return LDPC_OK;
+}
+
+static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
+    if (n < 0 || n > p->max_cw || (n > 0 && !cws)) {
+        set_last_error("mixed plan: codeword count outside 0..max_cw");
+        return LDPC_EINVAL;
+    }
+    int st = wait_upload(p);
+    if (st) return st;
+    p->planned = false;
+    std::map<int, std::vector<int>> groups;
+    for (int i = 0; i < n; i++) {
+        const ldpc_mixed_cw& d = cws[i];
+        MixedCode* mc = nullptr;
+        if ((st = mixed_get_code(p, d.kind, d.Z, &mc))) {
+            set_last_error("mixed plan: descriptor " + std::to_string(i) + ": " + ldpc_last_error());
+            return st;
+        }
+        const int rows_full = mc->code.rows_full;
+        if (d.m_rows < 4 || d.m_rows > rows_full || d.llr_offset < 0 || d.hd_offset < 0 || d.reserved != 0) {
+            set_last_error("mixed plan: descriptor " + std::to_string(i) + " has rows outside 4.." +
+                           std::to_string(rows_full) + " or a negative offset");
+            return LDPC_EINVAL;
+        }
+        groups[code_key(d.kind, d.Z)].push_back(i);
+    }
+    struct HostTile { MixedTile t; long long cost; int seq; };
+    std::vector<HostTile> tiles;
+    std::vector<int> slot_of;  // descriptor per slot, in group order
+    slot_of.reserve(n);
+    int trained_T = 0;
+    for (auto& g : groups) {
+        MixedCode& mc = p->codes[g.first];
+        std::vector<int>& v = g.second;
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return cws[a].m_rows > cws[b].m_rows; });
+        const int cpb = mixed_cpb(mc);
+        if (mc.d_weights) trained_T = mc.weights_iters;
+        for (size_t s = 0; s < v.size(); s += cpb) {
+            HostTile h{};
+            h.t.etab = mc.d_etab;
+            h.t.weights = mc.d_weights ? mc.d_weights : p->d_plain_w[mc.code.kind == kBG1 ? 0 : 1];
+            h.t.kind = mc.code.kind;
+            h.t.Z = mc.code.Z;
+            h.t.m_rows = cws[v[s]].m_rows;
+            h.t.count = (int)std::min<size_t>(cpb, v.size() - s);
+            h.t.cpb = h.t.count;  // a short last tile uses only the lanes it needs
+            h.t.first = (int)(slot_of.size());
+            h.t.lds_cw_bytes = mc.lds_cw;
+            for (int k = 0; k < h.t.count; k++) slot_of.push_back(v[s + k]);
+            const int waves = (h.t.cpb * mc.code.Z + 63) / 64;
+            h.cost = (long long)((waves + 3) / 4) * edges_in_rows(h.t.kind, h.t.m_rows);
+            h.seq = (int)tiles.size();
+            tiles.push_back(h);
+        }
+    }
+    std::stable_sort(tiles.begin(), tiles.end(), [](const HostTile& a, const HostTile& b) { return a.cost > b.cost; });
+    // staging: tiles in queue order, codewords in tile order
+    MixedTile* ht = reinterpret_cast<MixedTile*>(p->h_plan + p->tiles_off);
+    MixedCw* hc = reinterpret_cast<MixedCw*>(p->h_plan + p->cws_off);
+    p->order.assign(n, 0);
+    p->tiles6.clear();
+    int slot = 0;
+    for (size_t t = 0; t < tiles.size(); t++) {
+        MixedTile mt = tiles[t].t;
+        const int first_old = mt.first;
+        mt.first = slot;
+        for (int k = 0; k < mt.count; k++) {
+            const int i = slot_of[first_old + k];
+            const ldpc_mixed_cw& d = cws[i];
+            const int kb = d.kind == kBG1 ? 22 : 10;
+            MixedCw c{};
+            c.llr_off = d.llr_offset;
+            c.hd_off = d.hd_offset;
+            c.id = i;
+            c.m_rows = d.m_rows;
+            c.n_vn = (kb + d.m_rows) * d.Z;
+            c.hd_bytes = (c.n_vn + 7) / 8;
+            hc[slot] = c;
+            p->order[slot] = i;
+            slot++;
+        }
+        ht[t] = mt;
+        const int six[6] = {mt.kind, mt.Z, mt.m_rows, mt.cpb, mt.first, mt.count};
+        p->tiles6.insert(p->tiles6.end(), six, six + 6);
+    }
+    *reinterpret_cast<int*>(p->h_plan) = (int)tiles.size();
+    p->n_tiles = (int)tiles.size();
+    p->n_cw = n;
+    p->plan_trained_T = trained_T;
+    p->planned = true;
+    return LDPC_OK;
+}
+
+// Enqueue the plan upload (whole buffers when `full`, for graph capture) and the kernel.
+static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
+                         float* app_out, int* iters_out, bool full) {
+    const bool learned = (flags & LDPC_FLAG_LEARNED) != 0;
+    if (n_iter < 1 || (learned && n_iter >= kPlainRows)) {
+        set_last_error("mixed decode: n_iter outside 1..64");
+        return LDPC_EINVAL;
+    }
+    if (learned && p->plan_trained_T && p->plan_trained_T != n_iter) {
+        set_last_error("mixed decode: learned weights in the plan are for another iteration count");
+        return LDPC_EINVAL;
+    }
+    const size_t head = full ? p->cws_off : p->tiles_off + (size_t)p->n_tiles * sizeof(MixedTile);
+    const size_t body = (full ? (size_t)p->max_cw : (size_t)p->n_cw) * sizeof(MixedCw);
+    NRLDPC_HIP(hipMemcpyAsync(p->d_plan, p->h_plan, head, hipMemcpyHostToDevice, s));
+    if (body) NRLDPC_HIP(hipMemcpyAsync(p->d_plan + p->cws_off, p->h_plan + p->cws_off, body, hipMemcpyHostToDevice, s));
+    MixedArgs a{};
+    a.tiles = reinterpret_cast<const MixedTile*>(p->d_plan + p->tiles_off);
+    a.cws = reinterpret_cast<const MixedCw*>(p->d_plan + p->cws_off);
+    a.queue = p->d_queue;
+    a.llr = llr;
+    a.hd = hd;
+    a.app_out = app_out;
+    a.iters_out = iters_out;
+    a.n_tiles = reinterpret_cast<const int*>(p->d_plan);
+    a.n_iter = n_iter;
+    a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
+    a.slot_off = p->lds - 16;
+    return mixed_launch(learned, a, p->grid, p->threads, (size_t)p->lds, s);
+}
+
+}  // namespace nrldpc
+
+extern "C" {
+
+int ldpc_mixed_create(ldpc_mixed** out, int device, int max_cw) {
+    if (!out || max_cw < 1) return LDPC_EINVAL;
+    *out = nullptr;
+    ldpc_mixed* p = new ldpc_mixed();
+    p->device = device;
+    p->max_cw = max_cw;
+    p->tiles_off = kHdrBytes;
+    p->cws_off = (p->tiles_off + (size_t)max_cw * sizeof(MixedTile) + 255) & ~(size_t)255;
+    p->plan_bytes = p->cws_off + (size_t)max_cw * sizeof(MixedCw);
+    int st = LDPC_OK;
+    auto fail = [&](int code) { ldpc_mixed_destroy(p); return code; };
+    if (device < 0) {
+        p->lds = (kMaxLdsBytes - 1024) & ~15;
+        p->h_plan = static_cast<unsigned char*>(calloc(1, p->plan_bytes));
+        if (!p->h_plan) return fail(LDPC_ENOMEM);
+        *out = p;
+        return LDPC_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device >= n) {
+        set_last_error("no such HIP device");
+        return fail(LDPC_ENODEV);
+    }
+    if (hipSetDevice(device) != hipSuccess) return fail(LDPC_EHIP);
+    p->lds = max_lds_bytes() & ~15;
+    if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return fail(LDPC_EHIP);
+    const int occ = std::max(1, mixed_occupancy(false, p->threads, (size_t)p->lds));
+    p->grid = p->n_cu * occ;
+    if (hipHostMalloc(reinterpret_cast<void**>(&p->h_plan), p->plan_bytes, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&p->d_plan), p->plan_bytes) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&p->d_queue), 4 * sizeof(unsigned)) != hipSuccess) {
+        set_last_error("mixed: allocation failed");
+        return fail(LDPC_ENOMEM);
+    }
+    memset(p->h_plan, 0, p->plan_bytes);
+    if (hipMemset(p->d_queue, 0, 4 * sizeof(unsigned)) != hipSuccess ||
+        hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming) != hipSuccess)
+        return fail(LDPC_EHIP);
+    for (int k = 0; k < 2; k++) {
+        const int nnz = nnz_full(k == 0 ? kBG1 : kBG2);
+        std::vector<float2> w((size_t)kPlainRows * nnz, float2{0.75f, 0.0f});
+        for (int e = 0; e < nnz; e++) w[e] = float2{0.f, 0.f};
+        if (hipMalloc(reinterpret_cast<void**>(&p->d_plain_w[k]), w.size() * sizeof(float2)) != hipSuccess ||
+            hipMemcpy(p->d_plain_w[k], w.data(), w.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(LDPC_ENOMEM);
+    }
+    (void)st;
+    *out = p;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_destroy(ldpc_mixed* p) {
+    if (!p) return LDPC_EINVAL;
+    if (p->device >= 0) {
+        (void)hipSetDevice(p->device);
+        if (p->upload_pending) (void)hipEventSynchronize(p->uploaded);
+        if (p->exec) (void)hipGraphExecDestroy(p->exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
+        if (p->uploaded) (void)hipEventDestroy(p->uploaded);
+        for (auto& c : p->codes) {
+            if (c.second.d_etab) (void)hipFree(c.second.d_etab);
+            if (c.second.d_weights) (void)hipFree(c.second.d_weights);
+        }
+        for (int k = 0; k < 2; k++)
+            if (p->d_plain_w[k]) (void)hipFree(p->d_plain_w[k]);
+        if (p->d_plan) (void)hipFree(p->d_plan);
+        if (p->d_queue) (void)hipFree(p->d_queue);
+        if (p->h_plan) (void)hipHostFree(p->h_plan);
+    } else {
+        free(p->h_plan);
+    }
+    delete p;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_plan(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n_cw) {
+    if (!p) return LDPC_EINVAL;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->device >= 0) NRLDPC_HIP(hipSetDevice(p->device));
+    return mixed_plan_locked(p, cws, n_cw);
+}
+
+int ldpc_mixed_plan_info(const ldpc_mixed* p, int* n_tiles, int* threads, int* grid, int* lds_bytes) {
+    if (!p) return LDPC_EINVAL;
+    if (n_tiles) *n_tiles = p->planned ? p->n_tiles : 0;
+    if (threads) *threads = p->threads;
+    if (grid) *grid = p->grid;
+    if (lds_bytes) *lds_bytes = p->lds;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order) {
+    if (!p || (!out && max_tiles > 0)) return LDPC_EINVAL;
+    if (!p->planned) { set_last_error("mixed: no plan"); return LDPC_EINVAL; }
+    const int n = std::min(max_tiles, p->n_tiles);
+    if (n > 0) std::copy(p->tiles6.begin(), p->tiles6.begin() + 6 * n, out);
+    if (order) std::copy(p->order.begin(), p->order.end(), order);
+    return p->n_tiles;
+}
+
+int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
+                            float* app_out, int* iters_out) {
+    if (!p || !llr || !hd) return LDPC_EINVAL;
+    if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (!p->planned) { set_last_error("mixed: no plan"); return LDPC_EINVAL; }
+    NRLDPC_HIP(hipSetDevice(p->device));
+    if (p->n_tiles == 0) return LDPC_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int st = mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out, false);
+    if (st) return st;
+    NRLDPC_HIP(hipEventRecord(p->uploaded, s));
+    p->upload_pending = true;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
+                       float* app_out, int* iters_out, const float* llr_host, size_t llr_floats, uint8_t* hd_host,
+                       size_t hd_bytes) {
+    if (!p || !llr || !hd || !stream) return LDPC_EINVAL;
+    if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (!p->planned) { set_last_error("mixed: plan a batch before capturing"); return LDPC_EINVAL; }
+    NRLDPC_HIP(hipSetDevice(p->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
+    if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
+    NRLDPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    if (llr_host && llr_floats)
+        ce = hipMemcpyAsync(const_cast<float*>(llr), llr_host, llr_floats * sizeof(float), hipMemcpyHostToDevice, s);
+    int st = ce == hipSuccess ? mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out, true) : LDPC_EHIP;
+    if (st == LDPC_OK && hd_host && hd_bytes)
+        ce = hipMemcpyAsync(hd_host, hd, hd_bytes, hipMemcpyDeviceToHost, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(s, &g);
+    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
+    if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); NRLDPC_HIP(ce); }
+    NRLDPC_HIP(ee);
+    p->graph = g;
+    NRLDPC_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    return LDPC_OK;
+}
+
+int ldpc_mixed_replay(ldpc_mixed* p, void* stream) {
+    if (!p) return LDPC_EINVAL;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (!p->exec) { set_last_error("mixed: nothing captured"); return LDPC_EINVAL; }
+    if (!p->planned) { set_last_error("mixed: no plan"); return LDPC_EINVAL; }
+    NRLDPC_HIP(hipSetDevice(p->device));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    NRLDPC_HIP(hipGraphLaunch(p->exec, s));
+    NRLDPC_HIP(hipEventRecord(p->uploaded, s));
+    p->upload_pending = true;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_set_weights(ldpc_mixed* p, int kind, int Z, int n_iter, const float* w) {
+    if (!p || !w || n_iter < 1) return LDPC_EINVAL;
+    if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    NRLDPC_HIP(hipSetDevice(p->device));
+    MixedCode* mc = nullptr;
+    int st = mixed_get_code(p, kind, Z, &mc);
+    if (st) return st;
+    if ((st = wait_upload(p))) return st;
+    NRLDPC_HIP(hipDeviceSynchronize());  // no decode may still read the old table
+    const int nnz = nnz_full(kind);
+    std::vector<float> h((size_t)(n_iter + 1) * nnz * 2, 0.f);
+    std::copy(w, w + (size_t)n_iter * nnz * 2, h.begin() + (size_t)nnz * 2);
+    if (mc->d_weights && mc->weights_iters != n_iter) {
+        NRLDPC_HIP(hipFree(mc->d_weights));
+        mc->d_weights = nullptr;
+    }
+    if (!mc->d_weights) NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&mc->d_weights), h.size() * sizeof(float)));
+    NRLDPC_HIP(hipMemcpy(mc->d_weights, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    mc->weights_iters = n_iter;
+    p->planned = false;  // tiles hold weight pointers: plan again
+    return LDPC_OK;
+}
+
+int ldpc_mixed_set_cpb(ldpc_mixed* p, int kind, int Z, int cpb) {
+    if (!p || cpb < 0) return LDPC_EINVAL;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->device >= 0) NRLDPC_HIP(hipSetDevice(p->device));
+    MixedCode* mc = nullptr;
+    int st = mixed_get_code(p, kind, Z, &mc);
+    if (st) return st;
+    if (cpb > mc->cpb_max) {
+        set_last_error("codewords per workgroup exceed the workgroup size or LDS");
+        return LDPC_EINVAL;
+    }
+    mc->cpb = cpb;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_get_cpb(ldpc_mixed* p, int kind, int Z, int* cpb) {
+    if (!p || !cpb) return LDPC_EINVAL;
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->device >= 0) NRLDPC_HIP(hipSetDevice(p->device));
+    MixedCode* mc = nullptr;
+    int st = mixed_get_code(p, kind, Z, &mc);
+    if (st) return st;
+    *cpb = mixed_cpb(*mc);
+    return LDPC_OK;
+}
+
+int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, int* cand_cpb, float* us_per_cw,
+                        int max_candidates, int* n_candidates, int* best_cpb) {
+    if (!p || n_iter < 1 || n_cw < 1 || n_cw > p->max_cw) return LDPC_EINVAL;
+    if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    NRLDPC_HIP(hipSetDevice(p->device));
+    MixedCode* mc = nullptr;
+    int st = mixed_get_code(p, kind, Z, &mc);
+    if (st) return st;
+    const int nvn = mc->code.n_vn(), hb = mc->code.hd_bytes();
+    std::vector<ldpc_mixed_cw> d(n_cw);
+    for (int i = 0; i < n_cw; i++)
+        d[i] = ldpc_mixed_cw{kind, Z, mc->code.rows_full, 0, (long long)i * nvn, (long long)i * hb};
+    // Candidates: every count that fills one more wavefront, plus 1 and the maximum.  Without
+    // early exit the work is data independent, so all-zero LLRs time the real decoder.
+    std::vector<int> cand{1};
+    for (int k = 1; k <= p->threads / 64; k++) cand.push_back(std::max(1, std::min(mc->cpb_max, 64 * k / Z)));
+    cand.push_back(mc->cpb_max);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    float* llr = nullptr;
+    uint8_t* hd = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (llr) (void)hipFree(llr);
+        if (hd) (void)hipFree(hd);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    if (hipMalloc(reinterpret_cast<void**>(&llr), (size_t)n_cw * nvn * sizeof(float)) != hipSuccess ||
+        hipMalloc(reinterpret_cast<void**>(&hd), (size_t)n_cw * hb) != hipSuccess ||
+        hipMemset(llr, 0, (size_t)n_cw * nvn * sizeof(float)) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        set_last_error("autotune: allocation failed");
+        return LDPC_ENOMEM;
+    }
+    const int saved = mc->cpb;
+    int best = 1;
+    float best_us = 3.4e38f;
+    for (size_t c = 0; c < cand.size() && st == LDPC_OK; c++) {
+        mc->cpb = cand[c];
+        if ((st = mixed_plan_locked(p, d.data(), n_cw))) break;
+        float ms = 0.f;
+        for (int rep = 0; rep < 4 && st == LDPC_OK; rep++) {
+            if (rep == 1) (void)hipEventRecord(e0, nullptr);
+            st = mixed_enqueue(p, nullptr, llr, n_iter, 0, hd, nullptr, nullptr, false);
+        }
+        if (st) break;
+        (void)hipEventRecord(e1, nullptr);
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            st = LDPC_EHIP;
+            break;
+        }
+        const float us = ms * 1000.f / 3.f / (float)n_cw;
+        if ((int)c < max_candidates) {
+            if (us_per_cw) us_per_cw[c] = us;
+            if (cand_cpb) cand_cpb[c] = cand[c];
+        }
+        if (us < best_us) { best_us = us; best = cand[c]; }
+    }
+    (void)hipDeviceSynchronize();
+    cleanup();
+    p->planned = false;
+    if (st) { mc->cpb = saved; return st; }
+    mc->cpb = best;
+    if (n_candidates) *n_candidates = (int)cand.size();
+    if (best_cpb) *best_cpb = best;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_save_tuning(const ldpc_mixed* p, const char* path) {
+    if (!p || !path) return LDPC_EINVAL;
+    FILE* f = fopen(path, "w");
+    if (!f) { set_last_error(std::string("cannot write ") + path); return LDPC_EINVAL; }
+    fprintf(f, "# kind Z cpb (codewords per workgroup of the mixed-batch decoder)\n");
+    for (const auto& c : p->codes)
+        if (c.second.cpb > 0) fprintf(f, "%d %d %d\n", c.second.code.kind, c.second.code.Z, c.second.cpb);
+    fclose(f);
+    return LDPC_OK;
+}
+
+int ldpc_mixed_load_tuning(ldpc_mixed* p, const char* path) {
+    if (!p || !path) return LDPC_EINVAL;
+    FILE* f = fopen(path, "r");
+    if (!f) { set_last_error(std::string("cannot read ") + path); return LDPC_EINVAL; }
+    char line[256];
+    int st = LDPC_OK;
+    while (st == LDPC_OK && fgets(line, sizeof line, f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int kind = 0, Z = 0, cpb = 0;
+        if (sscanf(line, "%d %d %d", &kind, &Z, &cpb) != 3) {
+            set_last_error(std::string("malformed tuning line: ") + line);
+            st = LDPC_EINVAL;
+            break;
+        }
+        st = ldpc_mixed_set_cpb(p, kind, Z, cpb);
+    }
+    fclose(f);
+    return st;
+}
+
+}  // extern "C"

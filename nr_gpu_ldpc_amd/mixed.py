@@ -1,0 +1,102 @@
+"""Mixed-batch workloads: codewords of both NR base graphs at every lifting size and rate.
+
+A batch draws, per codeword, the base graph (BG1 or BG2 with equal probability), a lifting
+size uniform over the 51 sizes of TS 38.212 Table 5.3.2-1, a nominal rate uniform in
+[rate_lo, rate_hi] and an Eb/N0 uniform in [ebn0_lo, ebn0_hi].  The rate is realised by
+keeping m parity rows of the base graph (rate = kb / (kb + m - 2) after puncturing the first
+2Z columns):
+
+    BG1 (kb = 22): m = clamp(round(22 / R - 20), 4, 46)   rates 0.917 .. 1/3
+    BG2 (kb = 10): m = clamp(round(10 / R - 8), 4, 42)    rates 5/6 .. 1/5
+
+so nominal rates outside a base graph's range are clamped to its nearest row count; the
+realised rate of every codeword is returned with the batch.
+
+Channel LLRs come from the library's Philox channel: each codeword is a full-rate codeword of
+its (base graph, Z) whose first (kb + m) * Z decoder columns are kept (a prefix of a full-rate
+codeword satisfies the first m parity rows), transmitted with the noise level of its own
+realised rate.
+"""
+import numpy as np
+
+import nr_gpu_ldpc_amd as nr
+
+LIFTING_SIZES = sorted({a * (1 << j) for a in (2, 3, 5, 7, 9, 11, 13, 15) for j in range(8)
+                        if a * (1 << j) <= 384})
+assert len(LIFTING_SIZES) == 51
+
+KB = {nr.BG1: 22, nr.BG2: 10}
+ROWS = {nr.BG1: 46, nr.BG2: 42}
+COLS = {nr.BG1: 68, nr.BG2: 52}
+
+
+def rows_for_rate(kind, rate):
+    kb = KB[kind]
+    m = np.rint(kb / np.asarray(rate, dtype=np.float64) - (kb - 2)).astype(np.int64)
+    return np.clip(m, 4, ROWS[kind])
+
+
+def realised_rate(kind, m):
+    return KB[kind] / (KB[kind] + np.asarray(m) - 2.0)
+
+
+def draw_batch(n, seed, rate=(0.2, 8 / 9), ebn0=(0.0, 4.0), kinds=(nr.BG1, nr.BG2), sizes=None):
+    """Random mixed batch: dict of numpy arrays kind, Z, m_rows, rate, ebn0 and the descriptor
+    array (MIXED_CW_DTYPE) with codewords packed back to back in the LLR / HD buffers."""
+    rng = np.random.default_rng(seed)
+    sizes = LIFTING_SIZES if sizes is None else list(sizes)
+    kind = np.asarray(kinds)[rng.integers(0, len(kinds), n)]
+    Z = np.asarray(sizes)[rng.integers(0, len(sizes), n)]
+    r_nom = rng.uniform(rate[0], rate[1], n)
+    m = np.empty(n, dtype=np.int64)
+    for k in kinds:
+        sel = kind == k
+        m[sel] = rows_for_rate(k, r_nom[sel])
+    return describe(kind, Z, m, ebn0=rng.uniform(ebn0[0], ebn0[1], n))
+
+
+def describe(kind, Z, m, ebn0=None):
+    kind = np.asarray(kind, dtype=np.int64)
+    Z = np.asarray(Z, dtype=np.int64)
+    m = np.asarray(m, dtype=np.int64)
+    kb = np.where(kind == nr.BG1, 22, 10)
+    n_vn = (kb + m) * Z
+    hd_bytes = (n_vn + 7) // 8
+    d = np.zeros(len(kind), dtype=nr.MIXED_CW_DTYPE)
+    d["kind"], d["Z"], d["m_rows"] = kind, Z, m
+    d["llr_offset"] = np.concatenate([[0], np.cumsum(n_vn)[:-1]])
+    d["hd_offset"] = np.concatenate([[0], np.cumsum(hd_bytes)[:-1]])
+    rate = np.where(kind == nr.BG1, realised_rate(nr.BG1, m), realised_rate(nr.BG2, m))
+    return {"kind": kind, "Z": Z, "m_rows": m, "n_vn": n_vn, "hd_bytes": hd_bytes, "rate": rate,
+            "ebn0": ebn0, "info_bits": kb * Z, "desc": d,
+            "llr_floats": int(n_vn.sum()), "hd_total": int(hd_bytes.sum())}
+
+
+def make_channel(torch, batch, seed, device="cuda", contexts=None):
+    """Channel LLRs [llr_floats] and transmitted bits [llr_floats] (bytes) of a drawn batch,
+    generated on the GPU, one channel launch per (base graph, Z)."""
+    llr = torch.zeros(batch["llr_floats"], dtype=torch.float32, device=device)
+    bits = torch.zeros(batch["llr_floats"], dtype=torch.uint8, device=device)
+    contexts = {} if contexts is None else contexts
+    kind, Z, n_vn = batch["kind"], batch["Z"], batch["n_vn"]
+    off = batch["desc"]["llr_offset"]
+    for key in sorted(set(zip(kind.tolist(), Z.tolist()))):
+        idx = np.nonzero((kind == key[0]) & (Z == key[1]))[0]
+        code = contexts.get(key)
+        if code is None:
+            code = contexts[key] = nr.Code(key[0], key[1])
+        sig = np.array([nr.noise_stddev(float(batch["ebn0"][i]), float(batch["rate"][i])) for i in idx],
+                       dtype=np.float32)
+        d_sig = torch.from_numpy(sig).to(device)
+        tmp = torch.empty((len(idx), code.n_vn), dtype=torch.float32, device=device)
+        tbits = torch.empty((len(idx), code.n_vn), dtype=torch.uint8, device=device)
+        code.channel(seed, int(key[0]) << 40 | int(key[1]) << 24, len(idx), tmp.data_ptr(),
+                     sigma_ptr=d_sig.data_ptr(), cw_bits_ptr=tbits.data_ptr())
+        lens = n_vn[idx]
+        src = np.concatenate([k * code.n_vn + np.arange(L) for k, L in enumerate(lens)])
+        dst = np.concatenate([off[i] + np.arange(L) for i, L in zip(idx, lens)])
+        s_t = torch.from_numpy(src).to(device)
+        d_t = torch.from_numpy(dst).to(device)
+        llr[d_t] = tmp.reshape(-1)[s_t]
+        bits[d_t] = tbits.reshape(-1)[s_t]
+    return llr, bits

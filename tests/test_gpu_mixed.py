@@ -1,0 +1,115 @@
+"""Persistent mixed-batch decoder on the GPU: every codeword equals the single-code decoder of
+its (base graph, Z, parity rows) on the same LLRs -- hard decisions, APP and iteration counts
+bit for bit -- with and without early exit, through a captured graph, and after autotuning."""
+import numpy as np
+import pytest
+
+from nr_gpu_ldpc_amd import mixed as mx
+
+pytestmark = pytest.mark.gpu
+
+
+def _reference(nr, torch, b, llr, n_iter, flags, contexts):
+    """Decode every codeword with the single-code decoder (grouped by (kind, Z, m))."""
+    n = len(b["kind"])
+    hd = [None] * n
+    app = [None] * n
+    its = np.zeros(n, dtype=np.int32)
+    keys = sorted(set(zip(b["kind"].tolist(), b["Z"].tolist(), b["m_rows"].tolist())))
+    off = b["desc"]["llr_offset"]
+    for key in keys:
+        idx = np.nonzero((b["kind"] == key[0]) & (b["Z"] == key[1]) & (b["m_rows"] == key[2]))[0]
+        code = contexts.get(key) or contexts.setdefault(key, nr.Code(*key))
+        L = code.n_vn
+        x = torch.stack([llr[int(off[i]):int(off[i]) + L] for i in idx]).contiguous()
+        d_hd = torch.zeros((len(idx), code.hd_bytes), dtype=torch.uint8, device="cuda")
+        d_app = torch.zeros((len(idx), L), dtype=torch.float32, device="cuda")
+        d_it = torch.zeros(len(idx), dtype=torch.int32, device="cuda")
+        code.decode(x.data_ptr(), len(idx), n_iter, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(), flags=flags)
+        h, a, t = d_hd.cpu().numpy(), d_app.cpu().numpy(), d_it.cpu().numpy()
+        for k, i in enumerate(idx):
+            hd[i], app[i], its[i] = h[k], a[k], t[k]
+    return hd, app, its
+
+
+def _mixed(nr, torch, b, llr, n_iter, flags, m=None):
+    m = m or nr.Mixed(len(b["kind"]))
+    m.plan(b["desc"])
+    d_hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
+    d_app = torch.zeros(b["llr_floats"], dtype=torch.float32, device="cuda")
+    d_it = torch.zeros(len(b["kind"]), dtype=torch.int32, device="cuda")
+    m.decode(llr.data_ptr(), n_iter, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(), flags=flags)
+    torch.cuda.synchronize()
+    return d_hd.cpu().numpy(), d_app.cpu().numpy(), d_it.cpu().numpy()
+
+
+def _compare(b, got, want):
+    hd, app, its = got
+    rhd, rapp, rits = want
+    d = b["desc"]
+    for i in range(len(b["kind"])):
+        h0, nb = int(d["hd_offset"][i]), int(b["hd_bytes"][i])
+        a0, nv = int(d["llr_offset"][i]), int(b["n_vn"][i])
+        assert np.array_equal(hd[h0:h0 + nb], rhd[i]), f"hard decisions of codeword {i} {d[i]}"
+        assert np.array_equal(app[a0:a0 + nv].view(np.uint32), rapp[i].view(np.uint32)), f"APP of codeword {i}"
+    assert np.array_equal(its, rits)
+
+
+@pytest.mark.parametrize("early", [False, True])
+def test_mixed_matches_single_code_decoder(nr, torch_gpu, early):
+    torch = torch_gpu
+    print(f"case mixed vs single early={early}", flush=True)
+    sizes = [2, 3, 7, 12, 15, 26, 52, 60, 104, 144, 208, 240, 352, 384]
+    b = mx.draw_batch(240, seed=11, sizes=sizes, ebn0=(0.5, 3.0))
+    llr, _ = mx.make_channel(torch, b, seed=5)
+    flags = nr.EARLY_EXIT if early else 0
+    got = _mixed(nr, torch, b, llr, 12, flags)
+    want = _reference(nr, torch, b, llr, 12, flags, {})
+    _compare(b, got, want)
+    if early:
+        assert got[2].min() < 12
+
+
+def test_mixed_graph_replay_follows_the_plan(nr, torch_gpu):
+    """One captured graph (plan upload, LLR copy in, decode, HD copy out) replayed for two
+    different plans gives what direct decodes give."""
+    torch = torch_gpu
+    print("case mixed graph replay", flush=True)
+    b1 = mx.draw_batch(150, seed=21, sizes=[8, 24, 64, 208, 384])
+    b2 = mx.draw_batch(90, seed=22, sizes=[5, 40, 96, 384])
+    cap_f = max(b1["llr_floats"], b2["llr_floats"])
+    cap_h = max(b1["hd_total"], b2["hd_total"])
+    m = nr.Mixed(200)
+    d_llr = torch.zeros(cap_f, dtype=torch.float32, device="cuda")
+    d_hd = torch.zeros(cap_h, dtype=torch.uint8, device="cuda")
+    d_it = torch.zeros(200, dtype=torch.int32, device="cuda")
+    h_llr = torch.zeros(cap_f, dtype=torch.float32).pin_memory()
+    h_hd = torch.zeros(cap_h, dtype=torch.uint8).pin_memory()
+    s = torch.cuda.Stream()
+    m.plan(b1["desc"])
+    m.capture(s.cuda_stream, d_llr.data_ptr(), 10, d_hd.data_ptr(), 0, d_it.data_ptr(), flags=nr.EARLY_EXIT,
+              llr_host_ptr=h_llr.data_ptr(), llr_floats=cap_f, hd_host_ptr=h_hd.data_ptr(), hd_bytes=cap_h)
+    for rep, b in enumerate((b1, b2, b1)):
+        llr, _ = mx.make_channel(torch, b, seed=30 + rep)
+        torch.cuda.synchronize()
+        h_llr[:b["llr_floats"]].copy_(llr.cpu())
+        m.plan(b["desc"])
+        m.replay(s.cuda_stream)
+        s.synchronize()
+        want = _mixed(nr, torch, b, llr, 10, nr.EARLY_EXIT)
+        assert np.array_equal(h_hd.numpy()[:b["hd_total"]], want[0][:b["hd_total"]]), f"replay {rep}"
+        assert np.array_equal(d_it.cpu().numpy()[:len(b["kind"])], want[2]), f"replay {rep}"
+
+
+def test_autotune_picks_a_timed_candidate(nr, torch_gpu):
+    torch = torch_gpu
+    print("case mixed autotune BG2 Z=52", flush=True)
+    m = nr.Mixed(2048)
+    best, table = m.autotune(2, 52, 10, 2048)
+    assert best in table and all(t > 0 for t in table.values())
+    assert table[best] == min(table.values())
+    assert m.get_cpb(2, 52) == best
+    # decoding after tuning is still exact
+    b = mx.describe([2] * 30, [52] * 30, [42] * 15 + [10] * 15, ebn0=np.full(30, 1.0))
+    llr, _ = mx.make_channel(torch, b, seed=3)
+    _compare(b, _mixed(nr, torch, b, llr, 8, 0, m), _reference(nr, torch, b, llr, 8, 0, {}))
