@@ -1,11 +1,19 @@
 #!/usr/bin/env python3
-"""Benchmark of the decode hot path: flooding scaled min-sum, 5G NR BG1 Z=384, T=10 iterations,
-batches of 4096 codewords (info Gbps per GPU), with the kernel's roofline and a CPU baseline.
+"""Benchmark of the decode hot path on one GPU.
 
-A step = one decode launch over one batch of 4096 codewords whose LLRs are already resident
-in HBM (generated on the device by the Philox channel kernel before timing).
+Headline (``value``): flooding scaled min-sum, 5G NR BG1 Z=384, T=10 iterations, batches of
+4096 codewords, information Gbps.  A step = one decode launch over one batch whose LLRs are
+already resident in HBM (generated on the device by the Philox channel kernel before timing).
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Also measured, each in its own object of the same JSON line:
+  * ``mixed_batch``: 16384 codewords of random base graph / lifting size / rate / Eb/N0 in
+    one persistent launch, at most 20 iterations with early exit (autotuned tiles);
+  * ``combined_stream``: BG1 Z=384 and BG2 Z=208 codewords at mixed rates in batches of
+    1..8192, each batch a replayed HIP graph (LLR upload, plan upload, decode, hard-decision
+    download), plain min-sum against learned weights with plain fallback;
+  * ``cpu_baseline``: the C oracle (same schedule) on this host's cores.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--skip-mixed] [--skip-stream]
 
 Prints one JSON line.
 """
@@ -15,6 +23,8 @@ import os
 import statistics
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -28,6 +38,8 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 # Non-packed 32-bit VALU lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
 PEAK_VALU_GOPS = 256 * 64 * 2.4
 VALU_OPS_PER_EDGE = 8  # convention: sub, min, med3, sign, select, sign-apply, dt-sub, APP-add
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_decode_bg1_z384_t10_b4096_traffic.json")
+TUNING_FILE = os.path.join(ROOT, "nr_gpu_ldpc_amd", "tuning", "mixed_gfx950.txt")
 
 
 def parse():
@@ -35,11 +47,18 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=20)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--cpu-sample", type=int, default=1024, help="codewords in the CPU-baseline sample")
+    p.add_argument("--cpu-sample", type=int, default=4096, help="codewords in the CPU-baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--skip-mixed", action="store_true")
+    p.add_argument("--skip-stream", action="store_true")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 256 MiB device-to-device copy (scale factor for PMC byte counters)")
     return p.parse_args()
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, max(0, int(round(q * len(v))) - 1))]
 
 
 def copy_calibration():
@@ -65,7 +84,222 @@ def cpu_baseline(llr_host, n_cw):
     dt = time.perf_counter() - t0
     return {"value": round(n_cw * 22 * Z / dt / 1e9, 6), "unit": "Gbps", "cores": threads, "kind": "port",
             "sample": f"{n_cw} codewords of the same BG1 Z=384 T=10 workload, decoded by the C oracle "
-                      f"with {threads} threads ({dt:.2f} s wall)"}
+                      f"with {threads} threads ({dt:.2f} s wall, about {dt * threads:.0f} CPU-s)"}
+
+
+def traffic_per_launch():
+    """HBM bytes per decode launch measured by rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE,
+    each scaled by the counter's reading for a copy of known size), if profiled."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["bytes_per_launch"], t
+    except (OSError, ValueError, KeyError):
+        return None, None
+
+
+def bench_headline(torch, nr, args):
+    code = nr.Code(nr.BG1, Z)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    llr = torch.empty((BATCH, code.n_vn), dtype=torch.float32, device=dev)
+    bits = torch.empty((BATCH, code.n_vn), dtype=torch.uint8, device=dev)
+    hd = torch.zeros((BATCH, code.hd_bytes), dtype=torch.uint8, device=dev)
+    sigma = nr.noise_stddev(EBN0_DB, RATE)
+    code.channel(2024, 0, BATCH, llr.data_ptr(), sigma=sigma, cw_bits_ptr=bits.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev[0].record(stream)
+    for k in range(args.steps):
+        code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
+    ms = ev[0].elapsed_time(ev[-1]) / args.steps
+    gbps = BATCH * code.K * args.steps / (wall * 1e9)
+    counts = torch.zeros(2, dtype=torch.int64, device=dev)
+    code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, counts.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    bit_err, frame_err = counts.tolist()
+
+    edges = code.nnz * Z
+    valu_ops = BATCH * edges * N_ITER * VALU_OPS_PER_EDGE
+    alg_bytes = BATCH * (4 * code.n_vn + code.hd_bytes)
+    valu_gops = valu_ops / (ms * 1e6)
+    hbm_gbs = alg_bytes / (ms * 1e6)
+    traffic, tsrc = traffic_per_launch()
+    res = {
+        "value": round(gbps, 4), "ms_per_step": round(ms, 4),
+        "cw_per_block": code.cw_per_block,
+        "latency_ms": {"p50": round(statistics.median(per), 4), "p99": round(pct(per, 0.99), 4),
+                       "kind": "device time per batch of 4096"},
+        "fer": round(frame_err / BATCH, 6), "ber": bit_err / (BATCH * code.K),
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(hbm_gbs, 1),
+            "peak": PEAK_HBM_GBS,
+            "unit": "GB/s",
+            "frac": round(hbm_gbs / PEAK_HBM_GBS, 5),
+            "traffic": traffic,
+            "per_launch": {"algorithmic_bytes": alg_bytes, "bytes_per_codeword": 4 * code.n_vn + code.hd_bytes,
+                           "codewords": BATCH, "edge_iterations": BATCH * edges * N_ITER},
+            "traffic_source": (tsrc or {}).get("source"),
+            "valu": {"note": "the binding limit: min-sum is compare/select/add work on the VALU",
+                     "achieved": round(valu_gops, 1), "peak": round(PEAK_VALU_GOPS, 1), "unit": "Gop/s",
+                     "frac": round(valu_gops / PEAK_VALU_GOPS, 4), "ops_per_edge": VALU_OPS_PER_EDGE},
+        },
+    }
+    return res, llr
+
+
+def bench_mixed(torch, nr, args):
+    """16384 codewords: BG1/BG2 50/50, Z uniform over the 51 sizes, rate U[1/5, 8/9] by
+    parity-row truncation (clamped per base graph), Eb/N0 U[0, 4] dB, T <= 20, early exit."""
+    from nr_gpu_ldpc_amd import mixed as mx
+    n, T = 16384, 20
+    b = mx.draw_batch(n, seed=2025)
+    llr, bits = mx.make_channel(torch, b, seed=77)
+    m = nr.Mixed(n)
+    tuned = os.path.exists(TUNING_FILE)
+    if tuned:
+        m.load_tuning(TUNING_FILE)
+    m.plan(b["desc"])
+    hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
+    its = torch.zeros(n, dtype=torch.int32, device="cuda")
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    for _ in range(2):
+        m.decode(llr.data_ptr(), T, hd.data_ptr(), 0, its.data_ptr(), flags=nr.EARLY_EXIT, stream=s)
+    torch.cuda.synchronize()
+    steps = max(5, args.steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+    for k in range(steps):
+        ev[2 * k].record(stream)
+        m.decode(llr.data_ptr(), T, hd.data_ptr(), 0, its.data_ptr(), flags=nr.EARLY_EXIT, stream=s)
+        ev[2 * k + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [ev[2 * k].elapsed_time(ev[2 * k + 1]) for k in range(steps)]
+    # the same batch through a captured graph (plan upload + decode), replayed
+    gs = torch.cuda.Stream()
+    m.capture(gs.cuda_stream, llr.data_ptr(), T, hd.data_ptr(), 0, its.data_ptr(), flags=nr.EARLY_EXIT)
+    gev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * steps)]
+    for k in range(steps):
+        gev[2 * k].record(gs)
+        m.replay(gs.cuda_stream)
+        gev[2 * k + 1].record(gs)
+    torch.cuda.synchronize()
+    gper = [gev[2 * k].elapsed_time(gev[2 * k + 1]) for k in range(steps)]
+    be, fe, _ = mx.info_errors(torch, b, hd, bits)
+    info = int(b["info_bits"].sum())
+    tiles, _ = m.tiles()
+    cpb = {}
+    for kind, z, _, c, _, cnt in tiles.tolist():
+        key = f"{kind}/{z}"
+        cpb[key] = max(cpb.get(key, 0), c)
+    info_ = m.plan_info()
+    p50 = statistics.median(per)
+    return {
+        "workload": "16384 codewords in one persistent launch: BG1/BG2 50/50, Z uniform over the 51 "
+                    "TS 38.212 sizes, rate U[1/5, 8/9] by parity-row truncation (clamped per base graph), "
+                    "Eb/N0 U[0, 4] dB, at most 20 iterations with per-codeword early exit",
+        "info_gbps": round(info / (p50 * 1e6), 3),
+        "latency_ms": {"p50": round(p50, 3), "p99": round(pct(per, 0.99), 3), "kind": "device time per batch"},
+        "graph_latency_ms": {"p50": round(statistics.median(gper), 3), "p99": round(pct(gper, 0.99), 3),
+                             "kind": "replayed graph (plan upload + decode), device time"},
+        "mean_iterations": round(float(its.float().mean()), 3),
+        "fer": round(fe / n, 6), "ber": be / info,
+        "tiling": {"source": os.path.relpath(TUNING_FILE, ROOT) if tuned else "default (most codewords that fit)",
+                   "threads": info_["threads"], "grid": info_["grid"], "lds_bytes": info_["lds_bytes"],
+                   "tiles": info_["n_tiles"], "cpb_examples": {k: cpb[k] for k in sorted(cpb)[:6]},
+                   "cpb_bg1_384": cpb.get("1/384"), "cpb_bg2_208": cpb.get("2/208")},
+        "seed": {"batch": 2025, "channel": 77},
+    }
+
+
+def bench_stream(torch, nr, args):
+    """BG1 Z=384 / BG2 Z=208 at mixed rates, batches of 1..8192 codewords, each batch one
+    replayed graph: LLR upload, plan upload, decode, hard-decision download."""
+    from nr_gpu_ldpc_amd import mixed as mx
+    from nr_gpu_ldpc_amd import noms
+    T = 10
+    rng = np.random.default_rng(99)
+    weights = {}
+    for kind, z in ((nr.BG1, 384), (nr.BG2, 208)):
+        path = noms.weights_path(kind, z, T)
+        if os.path.exists(path):
+            weights[(kind, z)] = noms.load_weights(path)["weights"]
+    classes = [1 << k for k in range(14)]  # 1 .. 8192
+    max_f = 8192 * 68 * 384
+    max_h = 8192 * ((68 * 384 + 7) // 8)
+    d_llr = torch.zeros(max_f, dtype=torch.float32, device="cuda")
+    d_hd = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
+    h_llr = torch.zeros(max_f, dtype=torch.float32).pin_memory()
+    h_hd = torch.zeros(max_h, dtype=torch.uint8).pin_memory()
+    stream = torch.cuda.Stream()
+    s = stream.cuda_stream
+    modes = {"plain": nr.EARLY_EXIT, "learned": nr.EARLY_EXIT | nr.LEARNED | nr.FALLBACK_PLAIN}
+    objs = {}
+    for mode, flags in modes.items():
+        for c in classes:
+            m = nr.Mixed(c)
+            if mode == "learned":
+                for (kind, z), w in weights.items():
+                    m.set_weights(kind, z, w)
+            m.plan(mx.describe([nr.BG1] * c, [384] * c, [46] * c)["desc"])
+            m.capture(s, d_llr.data_ptr(), T, d_hd.data_ptr(), 0, 0, flags=flags, llr_host_ptr=h_llr.data_ptr(),
+                      llr_floats=c * 68 * 384, hd_host_ptr=h_hd.data_ptr(), hd_bytes=c * ((68 * 384 + 7) // 8))
+            objs[(mode, c)] = m
+    ebn0_points = (1.0, 1.5, 2.0)
+    batches = [(c, int(rng.integers(c // 2 + 1, c + 1)), ebn0_points[i % 3]) for i, c in enumerate(classes * 3)]
+    batches += [(8192, 8192, e) for e in ebn0_points for _ in range(2)]  # FER statistics
+    stats = {mode: {"lat": [], "info": 0, "time_ms": 0.0, "fer": {e: [0, 0] for e in ebn0_points}}
+             for mode in modes}
+    contexts = {}
+    for bi, (c, n, e) in enumerate(batches):
+        kind = np.where(rng.random(n) < 0.5, nr.BG1, nr.BG2)
+        Zs = np.where(kind == nr.BG1, 384, 208)
+        r_nom = rng.uniform(0.2, 8 / 9, n)
+        mr = np.where(kind == nr.BG1, mx.rows_for_rate(nr.BG1, r_nom), mx.rows_for_rate(nr.BG2, r_nom))
+        b = mx.describe(kind, Zs, mr, ebn0=np.full(n, e))
+        llr, bits = mx.make_channel(torch, b, seed=1000 + bi, contexts=contexts)
+        h_llr[:b["llr_floats"]].copy_(llr.cpu())
+        for mode in modes:
+            m = objs[(mode, c)]
+            m.plan(b["desc"])
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            m.replay(s)
+            e1.record(stream)
+            stream.synchronize()
+            ms = e0.elapsed_time(e1)
+            hd = h_hd[:b["hd_total"]].cuda()
+            _, fe, _ = mx.info_errors(torch, b, hd, bits)
+            st = stats[mode]
+            st["lat"].append(ms)
+            st["info"] += int(b["info_bits"].sum())
+            st["time_ms"] += ms
+            st["fer"][e][0] += fe
+            st["fer"][e][1] += n
+    out = {"workload": "BG1 Z=384 and BG2 Z=208 (50/50) at rates U[1/5, 8/9] by parity-row truncation, "
+                       "batches of 1..8192 codewords (sizes in (2^k/2, 2^k], 3 per power of two, plus 2 x 8192 "
+                       "per Eb/N0 point), Eb/N0 1.0 / 1.5 / 2.0 dB, T=10 with early exit; each batch is one "
+                       "replayed HIP graph: LLR upload, plan upload, persistent decode, hard-decision download",
+           "learned_weights": sorted(f"{k}/{z}" for k, z in weights)}
+    for mode, st in stats.items():
+        out[mode] = {"info_gbps": round(st["info"] / (st["time_ms"] * 1e6), 3),
+                     "latency_ms": {"p50": round(statistics.median(st["lat"]), 3),
+                                    "p99": round(pct(st["lat"], 0.99), 3),
+                                    "kind": "device time per batch incl. host copies"},
+                     "fer": {f"{e}dB": round(v[0] / max(1, v[1]), 6) for e, v in st["fer"].items()},
+                     "frames": {f"{e}dB": v[1] for e, v in st["fer"].items()}}
+    return out
 
 
 def main():
@@ -80,58 +314,15 @@ def main():
     import torch
     import nr_gpu_ldpc_amd as nr
 
-    code = nr.Code(nr.BG1, Z)
-    dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
-    s = stream.cuda_stream
-    llr = torch.empty((BATCH, code.n_vn), dtype=torch.float32, device=dev)
-    bits = torch.empty((BATCH, code.n_vn), dtype=torch.uint8, device=dev)
-    hd = torch.zeros((BATCH, code.hd_bytes), dtype=torch.uint8, device=dev)
-    sigma = nr.noise_stddev(EBN0_DB, RATE)
-    code.channel(2024, 0, BATCH, llr.data_ptr(), sigma=sigma, cw_bits_ptr=bits.data_ptr(), stream=s)
-    torch.cuda.synchronize()
-
-    for _ in range(args.warmup):
-        code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
-    torch.cuda.synchronize()
-
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ev[0].record(stream)
-    for k in range(args.steps):
-        code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
-        ev[k + 1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    per = [ev[k].elapsed_time(ev[k + 1]) for k in range(args.steps)]
-    total_ms = ev[0].elapsed_time(ev[-1])
-    ms = total_ms / args.steps
-    info_bits = BATCH * code.K
-    gbps = info_bits * args.steps / (wall * 1e9)
-
-    counts = torch.zeros(2, dtype=torch.int64, device=dev)
-    code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, counts.data_ptr(), stream=s)
-    torch.cuda.synchronize()
-    bit_err, frame_err = counts.tolist()
-
-    # roofline of the decode kernel (one launch = one batch)
-    edges = code.nnz * Z
-    valu_ops = BATCH * edges * N_ITER * VALU_OPS_PER_EDGE
-    alg_bytes = BATCH * (4 * code.n_vn + code.hd_bytes)
-    valu_gops = valu_ops / (ms * 1e6)
-    hbm_gbs = alg_bytes / (ms * 1e6)
-    p50 = statistics.median(per)
-    p99 = sorted(per)[max(0, int(round(0.99 * len(per))) - 1)]
-
+    head, llr = bench_headline(torch, nr, args)
     out = {
         "metric": "info Gbps/GPU at FER 1e-3, learned vs plain min-sum; p99 batch latency",
-        "value": round(gbps, 4),
+        "value": head["value"],
         "unit": "Gbps",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -141,26 +332,19 @@ def main():
         "config": {"workload": "5G NR BG1 Z=384 (K=8448, N=25344), flooding scaled min-sum, "
                                "10 fixed iterations, batch 4096",
                    "bg": 1, "Z": Z, "n_iter": N_ITER, "batch": BATCH, "ebn0_db": EBN0_DB,
-                   "cw_per_block": code.cw_per_block},
-        "latency_ms": {"p50": round(p50, 4), "p99": round(p99, 4), "kind": "device time per batch"},
-        "fer": round(frame_err / BATCH, 6),
-        "ber": bit_err / (BATCH * code.K),
-        "roofline": {
-            "bound": "valu",
-            "achieved": round(valu_gops, 1),
-            "peak": round(PEAK_VALU_GOPS, 1),
-            "unit": "Gop/s",
-            "frac": round(valu_gops / PEAK_VALU_GOPS, 4),
-            "traffic": None,
-            "per_launch": {"edge_iterations": BATCH * edges * N_ITER, "ops_per_edge": VALU_OPS_PER_EDGE,
-                           "algorithmic_bytes": alg_bytes},
-            "hbm": {"achieved": round(hbm_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": round(hbm_gbs / PEAK_HBM_GBS, 5)},
-        },
+                   "cw_per_block": head["cw_per_block"]},
+        "latency_ms": head["latency_ms"],
+        "fer": head["fer"],
+        "ber": head["ber"],
+        "roofline": head["roofline"],
     }
+    if not args.skip_mixed:
+        out["mixed_batch"] = bench_mixed(torch, nr, args)
+    if not args.skip_stream:
+        out["combined_stream"] = bench_stream(torch, nr, args)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(llr.cpu().numpy(), min(args.cpu_sample, BATCH))
-    print(json.dumps(out))
+    print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
 
 
 if __name__ == "__main__":

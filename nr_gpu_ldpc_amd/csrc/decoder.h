@@ -58,7 +58,7 @@ struct MixedTile {
     int cpb;                // LDS codeword slots (lanes used: cpb * Z)
     int first, count;       // codewords first .. first + count - 1 of the MixedCw list
     int lds_cw_bytes;
-    int pad;
+    int trained;            // 1: learned weights of this (kind, Z) are set (weights != null)
 };
 
 struct MixedArgs {
@@ -72,6 +72,7 @@ struct MixedArgs {
     const int* n_tiles;     // device word: tiles in the list (read by the kernel, so a captured
                             // launch serves any plan that fits the buffers)
     int n_iter, early_exit;
+    int fallback;           // learned tiles: re-decode syndrome failures with plain min-sum
     int slot_off;           // LDS byte offset of the work-queue broadcast word
 };
 

@@ -156,9 +156,15 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // are forced to +0.0, which leaves every APP bit pattern unchanged (APP is never -0.0).
 // Lanes at or above cpb*Z shadow lane (cpb-1, tid mod Z): they compute the same values and
 // skip the one non-idempotent store (the accumulator read-modify-write) and the outputs.
+// MIXED only: `sel` marks the lane's codeword for decoding; `final_check` also evaluates the
+// syndrome after the last iteration; `emit_only_ok` then writes outputs only for codewords
+// whose hard decision passes it.  Returns, per lane, whether its codeword converged (early
+// exit) or passed the final check.
 template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED>
-__device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
-                                             const MixedCw* __restrict__ mcw) {
+__device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
+                                             const MixedCw* __restrict__ mcw, const bool sel = true,
+                                             const bool final_check = false,
+                                             const bool emit_only_ok = false) {
     using S = Structure<KIND>;
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && (TRAIN || ZC > 0)), "mixed tiles use the any-Z plain/learned path");
@@ -178,7 +184,7 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
     bool live;
     int cwc;
     if constexpr (MIXED) {
-        live = !dead && cwl < a.n_cw;
+        live = !dead && cwl < a.n_cw && sel;
         cwc = cwl < a.n_cw ? cwl : a.n_cw - 1;  // index into mcw
     } else {
         live = cw < a.n_cw && (a.select == nullptr || a.select[cw] != 0);
@@ -199,7 +205,7 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
     const uint32_t wgFlag = lds_base() + (uint32_t)a.cpb * a.lds_cw_bytes;  // workgroup flag slot
     CEdge* et = (CEdge*)(a.etab);
     const int m_rows = a.m_rows;
-    if (a.select != nullptr && !block_any(live, wgFlag)) return;  // nothing to decode here
+    if (a.select != nullptr && !block_any(live, wgFlag)) return false;  // nothing to decode here
 
     constexpr int NW = S::NW;
     float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
@@ -237,6 +243,7 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
     __syncthreads();
 
     bool emitted = false;
+    bool final_ok = false;
     const bool early = a.early_exit != 0;
     const float2* __restrict__ wt = a.weights;  // learned: [(n_iter+1)][nnz] (alpha, beta), row 0 zero
 
@@ -457,7 +464,10 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
 #endif
         });
         __syncthreads();
-        if (t + 1 == a.n_iter) break;
+        const bool last = t + 1 == a.n_iter;
+        bool fin = false;  // MIXED: syndrome of the final hard decision
+        if constexpr (MIXED) fin = final_check && last;
+        if (last && !fin) break;
         // APP_new becomes the read copy of the next iteration.
         if (!idle) {
 #pragma unroll 2
@@ -467,9 +477,10 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
                 lds_st(rRd + j * 2 * Z4 + Z4, v);
             }
         }
-        if (early && r == 0 && !idle) lds_stu(flagAddr, 0u);
+        const bool check = early || fin;
+        if (check && r == 0 && !idle) lds_stu(flagAddr, 0u);
         __syncthreads();
-        if (early) {
+        if (check) {
             // syndrome of the current hard decision; any unsatisfied check keeps decoding
             uint32_t bad = 0u;
             if (!idle) static_for<0, ROWS>([&](auto IC) {
@@ -495,6 +506,10 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
             if (bad) lds_stu(flagAddr, 1u);
             __syncthreads();
             const bool conv = lds_ldu(flagAddr) == 0u;
+            if (fin) {
+                final_ok = conv;
+                break;
+            }
             const bool fresh = live && conv && !emitted;
             if (block_any(fresh, wgFlag)) {
                 // staging reuses the read copy of the converged codewords only
@@ -504,8 +519,10 @@ __device__ __forceinline__ void decode_block(const DecodeArgs& a, const int blk,
             if (!block_any(live && !emitted, wgFlag)) break;
         }
     }
-    if (block_any(live && !emitted, wgFlag))
-        emit(live && !emitted, t < a.n_iter ? t + 1 : a.n_iter);
+    bool out = live && !emitted;
+    if constexpr (MIXED) out = out && (!emit_only_ok || final_ok);
+    if (block_any(out, wgFlag)) emit(out, t < a.n_iter ? t + 1 : a.n_iter);
+    return emitted || final_ok;
 }
 
 template <int KIND, int ZC, bool LEARNED, bool TRAIN = false>

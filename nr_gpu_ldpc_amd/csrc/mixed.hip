@@ -42,8 +42,23 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         a.lds_cw_bytes = t->lds_cw_bytes;
         a.nnz = kind == kBG1 ? Structure<kBG1>::NNZ : Structure<kBG2>::NNZ;
         const MixedCw* cws = m.cws + t->first;
-        if (kind == kBG1) decode_block<kBG1, 0, LEARNED, false, true>(a, 0, cws);
-        else decode_block<kBG2, 0, LEARNED, false, true>(a, 0, cws);
+        if (LEARNED && t->trained) {
+            // learned weights; with fallback, the syndrome of the final hard decision decides
+            // which codewords plain min-sum decodes again (its output replaces the learned one
+            // only where it passes the check)
+            const bool fb = m.fallback != 0;
+            bool ok;
+            if (kind == kBG1) ok = decode_block<kBG1, 0, true, false, true>(a, 0, cws, true, fb, false);
+            else ok = decode_block<kBG2, 0, true, false, true>(a, 0, cws, true, fb, false);
+            const bool need = fb && (int)threadIdx.x < t->count * a.Z && !ok;
+            if (fb && block_any(need, slot)) {
+                if (kind == kBG1) decode_block<kBG1, 0, false, false, true>(a, 0, cws, need, true, true);
+                else decode_block<kBG2, 0, false, false, true>(a, 0, cws, need, true, true);
+            }
+        } else {
+            if (kind == kBG1) decode_block<kBG1, 0, false, false, true>(a, 0, cws);
+            else decode_block<kBG2, 0, false, false, true>(a, 0, cws);
+        }
     }
     if (threadIdx.x == 0) {
         __threadfence();

@@ -29,7 +29,6 @@
 namespace nrldpc {
 
 constexpr int kMixedThreads = 512;  // workgroup size of every mixed launch
-constexpr int kPlainRows = 65;      // rows of the alpha = 0.75 weight tables (n_iter <= 64)
 constexpr size_t kHdrBytes = 256;
 
 struct MixedCode {
@@ -64,7 +63,6 @@ struct ldpc_mixed {
     unsigned char* d_plan = nullptr;
     size_t tiles_off = kHdrBytes, cws_off = 0, plan_bytes = 0;
     unsigned* d_queue = nullptr;
-    float2* d_plain_w[2] = {nullptr, nullptr};
     int n_tiles = 0, n_cw = 0;
     std::vector<int> order, tiles6;
     int plan_trained_T = 0;
@@ -158,7 +156,8 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
         for (size_t s = 0; s < v.size(); s += cpb) {
             HostTile h{};
             h.t.etab = mc.d_etab;
-            h.t.weights = mc.d_weights ? mc.d_weights : p->d_plain_w[mc.code.kind == kBG1 ? 0 : 1];
+            h.t.weights = mc.d_weights;
+            h.t.trained = mc.d_weights ? 1 : 0;
             h.t.kind = mc.code.kind;
             h.t.Z = mc.code.Z;
             h.t.m_rows = cws[v[s]].m_rows;
@@ -215,8 +214,8 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
 static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
                          float* app_out, int* iters_out, bool full) {
     const bool learned = (flags & LDPC_FLAG_LEARNED) != 0;
-    if (n_iter < 1 || (learned && n_iter >= kPlainRows)) {
-        set_last_error("mixed decode: n_iter outside 1..64");
+    if (n_iter < 1) {
+        set_last_error("mixed decode: n_iter must be positive");
         return LDPC_EINVAL;
     }
     if (learned && p->plan_trained_T && p->plan_trained_T != n_iter) {
@@ -238,6 +237,7 @@ static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_i
     a.n_tiles = reinterpret_cast<const int*>(p->d_plan);
     a.n_iter = n_iter;
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
+    a.fallback = (learned && (flags & LDPC_FLAG_FALLBACK_PLAIN)) ? 1 : 0;
     a.slot_off = p->lds - 16;
     return mixed_launch(learned, a, p->grid, p->threads, (size_t)p->lds, s);
 }
@@ -285,14 +285,6 @@ int ldpc_mixed_create(ldpc_mixed** out, int device, int max_cw) {
     if (hipMemset(p->d_queue, 0, 4 * sizeof(unsigned)) != hipSuccess ||
         hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming) != hipSuccess)
         return fail(LDPC_EHIP);
-    for (int k = 0; k < 2; k++) {
-        const int nnz = nnz_full(k == 0 ? kBG1 : kBG2);
-        std::vector<float2> w((size_t)kPlainRows * nnz, float2{0.75f, 0.0f});
-        for (int e = 0; e < nnz; e++) w[e] = float2{0.f, 0.f};
-        if (hipMalloc(reinterpret_cast<void**>(&p->d_plain_w[k]), w.size() * sizeof(float2)) != hipSuccess ||
-            hipMemcpy(p->d_plain_w[k], w.data(), w.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(LDPC_ENOMEM);
-    }
     (void)st;
     *out = p;
     return LDPC_OK;
@@ -310,8 +302,6 @@ int ldpc_mixed_destroy(ldpc_mixed* p) {
             if (c.second.d_etab) (void)hipFree(c.second.d_etab);
             if (c.second.d_weights) (void)hipFree(c.second.d_weights);
         }
-        for (int k = 0; k < 2; k++)
-            if (p->d_plain_w[k]) (void)hipFree(p->d_plain_w[k]);
         if (p->d_plan) (void)hipFree(p->d_plan);
         if (p->d_queue) (void)hipFree(p->d_queue);
         if (p->h_plan) (void)hipHostFree(p->h_plan);

@@ -92,11 +92,33 @@ def make_channel(torch, batch, seed, device="cuda", contexts=None):
         tbits = torch.empty((len(idx), code.n_vn), dtype=torch.uint8, device=device)
         code.channel(seed, int(key[0]) << 40 | int(key[1]) << 24, len(idx), tmp.data_ptr(),
                      sigma_ptr=d_sig.data_ptr(), cw_bits_ptr=tbits.data_ptr())
-        lens = n_vn[idx]
-        src = np.concatenate([k * code.n_vn + np.arange(L) for k, L in enumerate(lens)])
-        dst = np.concatenate([off[i] + np.arange(L) for i, L in zip(idx, lens)])
-        s_t = torch.from_numpy(src).to(device)
-        d_t = torch.from_numpy(dst).to(device)
-        llr[d_t] = tmp.reshape(-1)[s_t]
-        bits[d_t] = tbits.reshape(-1)[s_t]
+        # keep each codeword's prefix, scattered to its own offset (index work on the device)
+        lens = torch.from_numpy(n_vn[idx].astype(np.int64)).to(device)
+        keep = torch.arange(code.n_vn, device=device)[None, :] < lens[:, None]
+        rows = torch.repeat_interleave(torch.arange(len(idx), device=device), lens)
+        k = torch.arange(rows.numel(), device=device) - (torch.cumsum(lens, 0) - lens)[rows]
+        dst = torch.from_numpy(off[idx].astype(np.int64)).to(device)[rows] + k
+        llr[dst] = tmp[keep]
+        bits[dst] = tbits[keep]
     return llr, bits
+
+
+def info_errors(torch, batch, hd, bits):
+    """Information-bit errors of a decoded mixed batch, counted on the device.
+
+    hd: packed hard decisions (uint8 tensor addressed by the descriptors' hd offsets);
+    bits: transmitted bits (uint8 tensor addressed by the LLR offsets).  The first kb * Z
+    columns of every codeword are compared (punctured information bits included).
+    Returns (bit_errors, frame_errors, per-codeword error counts as a tensor)."""
+    dev = hd.device
+    n = len(batch["kind"])
+    K = torch.from_numpy(np.asarray(batch["info_bits"], dtype=np.int64)).to(dev)
+    hd_off = torch.from_numpy(batch["desc"]["hd_offset"].astype(np.int64)).to(dev)
+    llr_off = torch.from_numpy(batch["desc"]["llr_offset"].astype(np.int64)).to(dev)
+    cw = torch.repeat_interleave(torch.arange(n, device=dev), K)
+    k = torch.arange(cw.numel(), device=dev) - (torch.cumsum(K, 0) - K)[cw]
+    hb = 8 * hd_off[cw] + k
+    got = (hd[hb >> 3].to(torch.int32) >> (hb & 7).to(torch.int32)) & 1
+    err = (got != bits[llr_off[cw] + k].to(torch.int32)).to(torch.int64)
+    per_cw = torch.zeros(n, dtype=torch.int64, device=dev).index_add_(0, cw, err)
+    return int(per_cw.sum()), int((per_cw > 0).sum()), per_cw

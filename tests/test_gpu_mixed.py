@@ -113,3 +113,58 @@ def test_autotune_picks_a_timed_candidate(nr, torch_gpu):
     b = mx.describe([2] * 30, [52] * 30, [42] * 15 + [10] * 15, ebn0=np.full(30, 1.0))
     llr, _ = mx.make_channel(torch, b, seed=3)
     _compare(b, _mixed(nr, torch, b, llr, 8, 0, m), _reference(nr, torch, b, llr, 8, 0, {}))
+
+
+def _learned_reference(nr, torch, b, llr, n_iter, flags, weights):
+    n = len(b["kind"])
+    hd, app, its = [None] * n, [None] * n, np.zeros(n, dtype=np.int32)
+    off = b["desc"]["llr_offset"]
+    for key in sorted(set(zip(b["kind"].tolist(), b["Z"].tolist(), b["m_rows"].tolist()))):
+        idx = np.nonzero((b["kind"] == key[0]) & (b["Z"] == key[1]) & (b["m_rows"] == key[2]))[0]
+        code = nr.Code(*key)
+        L = code.n_vn
+        x = torch.stack([llr[int(off[i]):int(off[i]) + L] for i in idx]).contiguous()
+        d_hd = torch.zeros((len(idx), code.hd_bytes), dtype=torch.uint8, device="cuda")
+        d_app = torch.zeros((len(idx), L), dtype=torch.float32, device="cuda")
+        d_it = torch.zeros(len(idx), dtype=torch.int32, device="cuda")
+        w = weights.get(key[:2])
+        if w is None:
+            code.decode(x.data_ptr(), len(idx), n_iter, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(),
+                        flags=flags & nr.EARLY_EXIT)
+        else:
+            code.set_weights(w)
+            code.decode_learned(x.data_ptr(), len(idx), n_iter, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(),
+                                flags=flags & (nr.EARLY_EXIT | nr.FALLBACK_PLAIN))
+        h, a, t = d_hd.cpu().numpy(), d_app.cpu().numpy(), d_it.cpu().numpy()
+        for k, i in enumerate(idx):
+            hd[i], app[i], its[i] = h[k], a[k], t[k]
+    return hd, app, its
+
+
+@pytest.mark.parametrize("flags", ["learned", "learned+fallback", "learned+fallback+early"])
+def test_mixed_learned_matches_single_code(nr, torch_gpu, flags):
+    """Codes with weights set decode with them (others plain min-sum); with fallback, the plain
+    re-decode inside the persistent launch replaces exactly the codewords the single-code
+    learned path replaces."""
+    torch = torch_gpu
+    print(f"case mixed learned {flags}", flush=True)
+    f = nr.LEARNED
+    if "fallback" in flags:
+        f |= nr.FALLBACK_PLAIN
+    if "early" in flags:
+        f |= nr.EARLY_EXIT
+    T = 8
+    rng = np.random.default_rng(4)
+    weights = {}
+    for key in ((1, 64), (2, 208), (2, 24)):
+        nnz = 316 if key[0] == 1 else 197
+        w = np.stack([rng.uniform(0.4, 1.0, (T, nnz)), rng.uniform(-0.2, 0.6, (T, nnz))], -1).astype(np.float32)
+        weights[key] = w
+    b = mx.draw_batch(160, seed=8, sizes=[24, 64, 208, 384], ebn0=(1.0, 3.0))
+    llr, _ = mx.make_channel(torch, b, seed=9)
+    m = nr.Mixed(len(b["kind"]))
+    for (k, z), w in weights.items():
+        m.set_weights(k, z, w)
+    got = _mixed(nr, torch, b, llr, T, f, m)
+    want = _learned_reference(nr, torch, b, llr, T, f, weights)
+    _compare(b, got, want)
