@@ -35,7 +35,7 @@ extern "C" {
 
 #define LDPC_FLAG_EARLY_EXIT 1u /* stop a codeword once its hard decision has zero syndrome */
 #define LDPC_FLAG_FALLBACK_PLAIN 2u /* learned decode: re-decode syndrome failures with plain min-sum */
-#define LDPC_FLAG_LEARNED 4u    /* mixed batches: learned weights where set, alpha = 0.75 elsewhere */
+#define LDPC_FLAG_LEARNED 4u    /* mixed batches: learned weights where set, plain min-sum elsewhere */
 
 typedef struct ldpc_ctx ldpc_ctx;
 
@@ -85,6 +85,13 @@ int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, i
  * No reference counterpart: the reference fixes n_cw_per_mcw at build time
  * (setup_gpu_grid src/setup_exp.cu:131-166). */
 int ldpc_set_launch(ldpc_ctx* ctx, int cw_per_block);
+
+/* Repack per-codeword packed decisions ([n_cw][hd_bytes], as ldpc_decode_async writes them) into
+ * one bit stream over the concatenated [n_cw][n_vn] columns: bit g of out (byte g/8, bit g%8) is
+ * column g % n_vn of codeword g / n_vn; out holds ceil(n_cw * n_vn / 8) bytes.  This is the
+ * layout of pack_hard_decision src/ldpc_kernels.cu:482-500 (identical to the per-codeword
+ * layout when n_vn % 8 == 0). */
+int ldpc_hd_to_stream_async(ldpc_ctx* ctx, void* stream, const uint8_t* hd_packed, int n_cw, uint8_t* out);
 
 /* Syndrome weight (unsatisfied checks) of packed hard decisions, one int per codeword.
  * No reference counterpart (early termination is a TODO in the reference, README.md:193). */
@@ -213,7 +220,10 @@ int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order);
 
 /* Decode the planned batch: llr / hd / app_out are device buffers addressed by the descriptors'
  * offsets; iters_out[i] (device, may be NULL) is the iteration count of descriptor i.
- * flags: LDPC_FLAG_EARLY_EXIT, LDPC_FLAG_LEARNED. */
+ * flags: LDPC_FLAG_EARLY_EXIT; LDPC_FLAG_LEARNED (codes with weights set use them, the others
+ * plain min-sum); with LDPC_FLAG_LEARNED, LDPC_FLAG_FALLBACK_PLAIN re-decodes, inside the same
+ * launch, the learned codewords whose final hard decision fails the syndrome check with plain
+ * min-sum and keeps the plain result where it passes (as ldpc_decode_learned_async). */
 int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags,
                             uint8_t* hd, float* app_out, int* iters_out);
 
@@ -226,7 +236,8 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
                        uint8_t* hd_host, size_t hd_bytes);
 int ldpc_mixed_replay(ldpc_mixed* p, void* stream);
 
-/* Learned weights of one (kind, Z) for LDPC_FLAG_LEARNED decodes: host [n_iter][nnz_full][2]. */
+/* Learned weights of one (kind, Z) for LDPC_FLAG_LEARNED decodes: host [n_iter][nnz_full][2]
+ * (rows beyond a codeword's parity-row count are unused).  Plan again after setting them. */
 int ldpc_mixed_set_weights(ldpc_mixed* p, int kind, int Z, int n_iter, const float* weights);
 
 /* Codewords per workgroup (LDS codeword slots) of (kind, Z); 0 = default (the most that fit).

@@ -1,0 +1,193 @@
+// nr_gpu_ldpc.hpp -- the reference's C++ host API for the decode path, over libnrldpc.so.
+//
+// Same names, argument meaning and call sequence as the reference's host interface, so a
+// program written against it (setup_ldpc -> setup_host -> setup_dev_memory -> setup_gpu_grid ->
+// put_h_compact_and_params_in_constant_memory -> generate_new_codewords ->
+// perform_gpu_ldpc_decoding -> unpack_char / cuda_error_check) builds against this header and
+// links libnrldpc.so instead.  Every GPU operation goes through the C ABI of ldpc_api.h.
+//
+//   reference                         declared at
+//   perform_gpu_ldpc_decoding<T>      src/gpu_decoder_top.h:13-18
+//   setup_host / setup_dev_memory /   src/setup_exp.h:13-30
+//     setup_gpu_grid / generate_new_codewords / put_h_compact_and_params_in_constant_memory
+//   setup_ldpc / ldpc_encoder_orig    src/nr_ldpc.h:14-20
+//   compute_noise_stddev / transmit / calculate_llr / error_check / cuda_error_check
+//                                     src/encoding_tools.h:12-28
+//   quantize / unpack_char            src/host_helpers.cpp:46-64
+//   types                             src/types.h:16-142
+//
+// Differences, all deliberate:
+//   * T = float is the instantiated element type (FP32 decode); char and half are not provided.
+//   * dev_mem::dt and dev_mem::r stay NULL: the decoder keeps its check-node state on chip.
+//   * Errors: HIP failures print the library's message and exit(EXIT_FAILURE), as the
+//     reference's checkCudaErrors does; invalid configurations print and abort().
+//   * The code description lives in a process-wide decoder context (one code per process, as
+//     the reference's __constant__ memory); put_h_compact_and_params_in_constant_memory
+//     replaces it.
+//   * Packed hard decisions are exact for every Z (the reference's packer assumes Z % 8 == 0);
+//     host_mem::mem_size_hd_cuda_packed is ceil(bits / 8).
+//   * ldpc_encoder_orig(gen_code = 1) (OAI code generation to files) is not provided.
+#ifndef NRLDPC_NR_GPU_LDPC_HPP
+#define NRLDPC_NR_GPU_LDPC_HPP
+
+#include <hip/hip_runtime.h>
+
+#include "ldpc_api.h"
+
+struct error_result {
+    int bit_error;
+    int n_bits;
+    int frame_error;
+    int n_frames;
+};
+inline error_result operator+(const error_result& a, const error_result& b) {
+    return {a.bit_error + b.bit_error, a.n_bits + b.n_bits, a.frame_error + b.frame_error, a.n_frames + b.n_frames};
+}
+
+struct h_element {
+    char x;
+    char y;
+    short value;
+    char valid;
+};
+
+struct ldpc_params {
+    int A;                          // payload size before CRC
+    float rate;                     // nominal code rate
+    int bg_index;                   // 1 or 2 (NR base graph); 3 = the 12 x 24 QC code (extension)
+    int Z;                          // lifting size
+    int K;                          // information bits per code block (incl. punctured)
+    int N;                          // transmitted bits
+    int H_rows;                     // base graph rows
+    int H_cols;                     // base graph columns
+    int* H;                         // raw base matrix [H_rows][H_cols], -1 = zero block
+    int n_nz_in_row;                // largest row degree
+    int n_nz_in_col;                // largest column degree
+    char* h_element_count_per_row;  // row degrees
+    char* h_element_count_per_col;  // column degrees
+    int n_iterations;
+    int N_before_puncture;          // decoder columns = H_cols * Z
+};
+
+struct params {
+    float snr_min;
+    float snr_max;
+    float snr_step;
+    int n_codewords;
+    int n_info_bits;
+    float code_rate;
+    int n_cw_per_mcw;
+    int n_mcw;
+    int n_streams;
+};
+
+template <typename T>
+struct host_mem {
+    int* info_bin;
+    unsigned long mem_size_infobits;
+    int* cwds;
+    float* modulated_cwds;
+    unsigned long mem_size_cw;
+    float* llr;
+    unsigned long mem_size_llr;
+    float* recv;
+    T* llr_quan;
+    // per-stream pinned staging
+    int** info_bin_cuda;
+    T** llr_cuda;
+    int mem_size_llr_cuda;
+    char** hd_cuda;
+    int mem_size_hd_cuda;
+    char** hd_packed_cuda;
+    int mem_size_hd_cuda_packed;
+};
+
+template <typename T>
+struct dev_mem {
+    hipStream_t* streams;
+    T** llr;
+    T** dt;                      // NULL (no message arrays in HBM)
+    T** r;                       // NULL
+    char** hard_decision;        // per-codeword packed decisions (used when the stream packing differs)
+    char** hard_decision_packed; // decisions packed over the concatenated [cw][N_before_puncture] stream
+};
+
+struct gpu_result {
+    float best_latency;
+    float cpu_run_time;
+    int n_bits_total;
+    int n_bit_errors;
+    int n_frames;
+    int n_frame_errors;
+};
+inline gpu_result operator+(const gpu_result& a, const gpu_result& b) {
+    return {a.best_latency < b.best_latency ? a.best_latency : b.best_latency, a.cpu_run_time + b.cpu_run_time,
+            a.n_bits_total + b.n_bits_total, a.n_bit_errors + b.n_bit_errors, a.n_frames + b.n_frames,
+            a.n_frame_errors + b.n_frame_errors};
+}
+
+// Geometry of one decode call.  The reference's fields keep their meaning for the batch shape
+// (cnp_blocks.y = macro-codewords, cnp_threads.y = codewords per macro-codeword); the fused
+// launch's own shape is in cnp_threads_per_block (lanes) and shared_r_cache_size (LDS bytes).
+struct cuda_grid {
+    dim3 cnp_blocks;
+    dim3 cnp_threads;
+    dim3 vnp_blocks;
+    dim3 vnp_threads;
+    dim3 pack_blocks;
+    dim3 pack_threads;
+    dim3 early_blocks;
+    dim3 early_threads;
+    unsigned long shared_r_cache_size;
+    unsigned int cnp_threads_per_block;
+};
+
+struct kernel_params {
+    unsigned int z;
+    unsigned int n_cw_per_mcw;
+    unsigned int n_total_vn;
+    unsigned int n_total_cn;
+    unsigned int n_blk_col;
+    unsigned int threads_per_block;
+};
+
+// ---- code setup and host-side channel (src/nr_ldpc.h, src/encoding_tools.h, src/host_helpers.cpp)
+ldpc_params setup_ldpc(int n_bits, float code_rate, int n_iterations);
+// Extension: the 12 x 24 QC example code at lifting size Z (no puncturing).
+ldpc_params setup_ldpc_qc12x24(int Z, int n_iterations);
+int ldpc_encoder_orig(int* test_input, int* channel_input, short block_length, short BG, unsigned char gen_code);
+short* choose_generator_matrix(short BG, short Zc);
+float compute_noise_stddev(float snr, float code_rate, int modulation = 2);
+void transmit(int* info_bin, int* codeword, float* modulate_cwds, unsigned long mem_size_codeword, float* recv,
+              int n_info_bits, int n_coded_bits, float sigma, int bg_index);
+void calculate_llr(float llr[], float recv[], int n_symbols, float sigma, int z);
+int error_check(float trans[], float recv[], int n_symbols);
+error_result cuda_error_check(int info[], char hard_decision[], int n_total_cws, int codeword_len, int info_length);
+signed char quantize(double D, double x, unsigned char B);
+void unpack_char(char* packed_input, char* unpacked_output, int size);
+
+// ---- memory, grid and the decode entry point (src/setup_exp.h, src/gpu_decoder_top.h)
+template <typename T>
+host_mem<T> setup_host(const params* p, unsigned int n_info_bits, unsigned int n_cw_bits);
+template <typename T>
+dev_mem<T> setup_dev_memory(const params* p, const ldpc_params* ldpc, const host_mem<T>* host);
+cuda_grid setup_gpu_grid(const ldpc_params* ldpc, int n_cw_per_mcw, int n_mcw);
+template <typename T>
+void generate_new_codewords(const params* p, host_mem<T>* h, const ldpc_params* l, int codewords_in_iteration,
+                            float sigma);
+void put_h_compact_and_params_in_constant_memory(const ldpc_params* ldpc, const params* p, const cuda_grid* g);
+template <typename T>
+void perform_gpu_ldpc_decoding(dev_mem<T>* d, host_mem<T>* h, const cuda_grid* g, int n_iterations,
+                               int stream_index);
+
+// The process-wide decoder context created by put_h_compact_and_params_in_constant_memory
+// (NULL before), for callers that mix this API with the C ABI.
+ldpc_ctx* nr_gpu_ldpc_context();
+
+extern template host_mem<float> setup_host<float>(const params*, unsigned int, unsigned int);
+extern template dev_mem<float> setup_dev_memory<float>(const params*, const ldpc_params*, const host_mem<float>*);
+extern template void generate_new_codewords<float>(const params*, host_mem<float>*, const ldpc_params*, int, float);
+extern template void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, const cuda_grid*, int,
+                                                      int);
+
+#endif

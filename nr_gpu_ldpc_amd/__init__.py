@@ -78,6 +78,7 @@ _sig("ldpc_ctx_info", ctypes.c_int, _P, ctypes.POINTER(CodeInfo))
 _sig("ldpc_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_base_matrix", ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_int)
 _sig("ldpc_setup_params", ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_int_p)
 _sig("ldpc_encode_host", ctypes.c_int, _P, _P, ctypes.c_int, _P)
@@ -204,6 +205,9 @@ class Code:
 
     def syndrome(self, hd_ptr, n_cw, weight_ptr, stream=0):
         _check(lib.ldpc_syndrome_async(self._h, stream, hd_ptr, n_cw, weight_ptr), "ldpc_syndrome_async")
+
+    def hd_to_stream(self, hd_ptr, n_cw, out_ptr, stream=0):
+        _check(lib.ldpc_hd_to_stream_async(self._h, stream, hd_ptr, n_cw, out_ptr), "ldpc_hd_to_stream_async")
 
     def channel(self, seed, stream_base, n_cw, llr_ptr, sigma=None, sigma_ptr=0, cw_bits_ptr=0, stream=0):
         _check(lib.ldpc_channel_async(self._h, stream, seed, stream_base, n_cw, sigma_ptr or None,

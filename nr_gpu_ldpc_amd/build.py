@@ -37,7 +37,9 @@ COMMON = [
 ] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
 SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
-           "encoder.cpp", "noms_api.cpp", "compat.cpp", "driver.cpp", "mixed_plan.cpp", "mixed.hip"]
+           "encoder.cpp", "noms_api.cpp", "compat.cpp", "mixed_plan.cpp", "mixed.hip"]
+DRIVER_SRC = os.path.join(HERE, "driver", "ldpc_ber.cpp")
+DRIVER = os.path.join(HERE, "bin", "ldpc_ber")
 
 
 def _compile(src):
@@ -71,7 +73,23 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print("built", os.path.relpath(LIB, ROOT))
+    build_driver(verbose)
     return LIB
+
+
+def build_driver(verbose=True):
+    """The BER/FER driver executable, linked against the library with an rpath to lib/."""
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}", DRIVER_SRC,
+           f"-L{LIBDIR}", "-lnrldpc", "-Wl,-rpath,$ORIGIN/../lib", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, DRIVER)
+    if verbose:
+        print("built", os.path.relpath(DRIVER, ROOT))
+    return DRIVER
 
 
 if __name__ == "__main__":

@@ -385,4 +385,18 @@ int ldpc_count_errors_async(ldpc_ctx* x, void* stream, const uint8_t* hd, const 
     return count_errors_launch(a, n_cw, static_cast<hipStream_t>(stream));
 }
 
+int ldpc_hd_to_stream_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, uint8_t* out) {
+    if (!x || !hd || !out || n_cw < 0) return LDPC_EINVAL;
+    if (n_cw == 0) return LDPC_OK;
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    StreamPackArgs a{};
+    a.hd = hd;
+    a.out = out;
+    a.hd_bytes = x->code.hd_bytes();
+    a.n_vn = x->code.n_vn();
+    a.n_bits = (long long)n_cw * x->code.n_vn();
+    return stream_pack_launch(a, static_cast<hipStream_t>(stream));
+}
+
 }  // extern "C"

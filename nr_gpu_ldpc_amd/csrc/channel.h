@@ -37,7 +37,16 @@ struct ErrorArgs {
     long long* counts;
 };
 
+// Per-codeword packed decisions -> one bit stream over the concatenated [cw][n_vn] columns.
+struct StreamPackArgs {
+    const uint8_t* hd;      // [n_cw][hd_bytes]
+    uint8_t* out;           // ceil(n_cw * n_vn / 8) bytes
+    int hd_bytes, n_vn;
+    long long n_bits;       // n_cw * n_vn
+};
+
 int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s);
+int stream_pack_launch(const StreamPackArgs& a, hipStream_t s);
 int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s);
 int count_errors_launch(const ErrorArgs& a, int n_cw, hipStream_t s);
 

@@ -218,6 +218,33 @@ __global__ void __launch_bounds__(256) k_count_errors(ErrorArgs a) {
     }
 }
 
+// One thread per output byte: bit k of byte b is column g % n_vn of codeword g / n_vn, g = 8b + k.
+__global__ void __launch_bounds__(256) k_stream_pack(StreamPackArgs a) {
+    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (8 * b >= a.n_bits) return;
+    uint32_t v = 0;
+    for (int k = 0; k < 8; k++) {
+        const long long g = 8 * b + k;
+        if (g >= a.n_bits) break;
+        const long long cw = g / a.n_vn;
+        const int c = (int)(g - cw * a.n_vn);
+        v |= ((a.hd[cw * a.hd_bytes + (c >> 3)] >> (c & 7)) & 1u) << k;
+    }
+    a.out[b] = (uint8_t)v;
+}
+
+int stream_pack_launch(const StreamPackArgs& a, hipStream_t s) {
+    const long long bytes = (a.n_bits + 7) / 8;
+    const long long blocks = (bytes + 255) / 256;
+    if (blocks > 0x7fffffffLL) {
+        set_last_error("stream pack: batch too large");
+        return LDPC_EINVAL;
+    }
+    hipLaunchKernelGGL(k_stream_pack, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
 int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s) {
     const size_t lds = ((size_t)a.n_vn + 15) & ~(size_t)15;
     hipLaunchKernelGGL(k_channel, dim3(n_cw), dim3(256), lds, s, a);
