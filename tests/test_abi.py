@@ -41,3 +41,28 @@ def test_host_only_context_refuses_device_work(nr):
         assert e.status == -4
     else:
         raise AssertionError("decode on a host-only context must fail")
+
+
+def test_cpp_host_api_exported(nr):
+    """Every function of include/nr_gpu_ldpc.hpp (the reference-named C++ API) is in the library."""
+    import shutil
+    import subprocess
+    nm = shutil.which("nm")
+    if nm is None:
+        import pytest
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-DC", "--defined-only", nr.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    want = ["setup_ldpc(int, float, int)", "setup_ldpc_qc12x24(int, int)",
+            "ldpc_encoder_orig(int*, int*, short, short, unsigned char)", "choose_generator_matrix(short, short)",
+            "compute_noise_stddev(float, float, int)", "transmit(int*, int*, float*, unsigned long, float*, int, int, float, int)",
+            "calculate_llr(float*, float*, int, float, int)", "error_check(float*, float*, int)",
+            "cuda_error_check(int*, char*, int, int, int)", "quantize(double, double, unsigned char)",
+            "unpack_char(char*, char*, int)", "host_mem<float> setup_host<float>(params const*, unsigned int, unsigned int)",
+            "dev_mem<float> setup_dev_memory<float>(params const*, ldpc_params const*, host_mem<float> const*)",
+            "setup_gpu_grid(ldpc_params const*, int, int)",
+            "void generate_new_codewords<float>(params const*, host_mem<float>*, ldpc_params const*, int, float)",
+            "put_h_compact_and_params_in_constant_memory(ldpc_params const*, params const*, cuda_grid const*)",
+            "void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, cuda_grid const*, int, int)",
+            "nr_gpu_ldpc_context()"]
+    missing = [w for w in want if w not in out]
+    assert not missing, missing

@@ -197,3 +197,29 @@ def test_noms_gradient_matches_finite_differences(oracle, nr):
             fd = (lp - lm) / (2 * h)
             assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(fd), (i, fd, g[i])
     assert np.abs(ga).sum() > 0 and np.abs(gb).sum() > 0
+
+
+@pytest.mark.parametrize("kind,Z,m", [(OR_WIFI, 5, 0), (OR_BG1, 3, 0), (OR_BG2, 4, 9)])
+def test_syndrome_is_dense_parity_check(oracle, kind, Z, m):
+    """oracle_syndrome = weight of H x over GF(2) with H expanded densely (brute force)."""
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    rng = np.random.default_rng(Z + 7 * kind)
+    for _ in range(5):
+        x = rng.integers(0, 2, H.shape[1], dtype=np.uint8)
+        assert oracle.syndrome(code, x) == int(((H.astype(np.int64) @ x) % 2).sum())
+    assert oracle.syndrome(code, np.zeros(H.shape[1], np.uint8)) == 0
+
+
+def test_llr_closed_form(oracle):
+    """oracle_llr: 2z punctured zeros, then (y * 2) / (sigma * sigma) in float32."""
+    import ctypes
+    recv = np.array([0.5, -1.25, 2.0, 0.0, -0.75, 1.0, 3.5], dtype=F)
+    out = np.full(9, np.nan, dtype=F)
+    oracle.lib.oracle_llr(recv.ctypes.data, 9, ctypes.c_float(0.8), 1, out.ctypes.data)
+    s = F(0.8)
+    want = np.concatenate([[0, 0], (recv * F(2)) / (s * s)]).astype(F)
+    assert np.array_equal(out, want)
+    out2 = np.full(4, np.nan, dtype=F)
+    oracle.lib.oracle_llr(recv.ctypes.data, 4, ctypes.c_float(0.5), 0, out2.ctypes.data)
+    assert out2.tolist() == [4.0, -10.0, 16.0, 0.0]  # sigma = 0.5: LLR = 8 y exactly
