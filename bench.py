@@ -309,7 +309,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the decode path would shard "
-                                   "codewords across GPUs (see DESIGN.md), run with --gpus 1"}))
+                                   "codewords across GPUs by codeword range; run with --gpus 1"}))
         sys.exit(2)
     import torch
     import nr_gpu_ldpc_amd as nr
