@@ -73,8 +73,20 @@ def copy_calibration():
                       "launches": 3}))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(llr_host, n_cw):
-    """The oracle (the reference's min-sum restated in C) on this host's cores."""
+    """The oracle (the reference's min-sum restated in C) on this host's cores: the whole
+    sample with one thread per core of the box's share, and a slice with one thread."""
     from tests import oracle_binding
     orc = oracle_binding.load()
     code = orc.code(oracle_binding.OR_BG1, Z)
@@ -82,9 +94,15 @@ def cpu_baseline(llr_host, n_cw):
     t0 = time.perf_counter()
     orc.decode(code, llr_host[:n_cw], N_ITER, threads=threads)
     dt = time.perf_counter() - t0
+    n1 = max(1, n_cw // 16)
+    t0 = time.perf_counter()
+    orc.decode(code, llr_host[:n1], N_ITER, threads=1)
+    dt1 = time.perf_counter() - t0
     return {"value": round(n_cw * 22 * Z / dt / 1e9, 6), "unit": "Gbps", "cores": threads, "kind": "port",
             "sample": f"{n_cw} codewords of the same BG1 Z=384 T=10 workload, decoded by the C oracle "
-                      f"with {threads} threads ({dt:.2f} s wall, about {dt * threads:.0f} CPU-s)"}
+                      f"with {threads} threads ({dt:.2f} s wall, about {dt * threads:.0f} CPU-s)",
+            "single_thread_gbps": round(n1 * 22 * Z / dt1 / 1e9, 6),
+            "host": {"cpu": cpu_model(), "nproc": os.cpu_count()}}
 
 
 def traffic_per_launch():

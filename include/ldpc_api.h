@@ -227,10 +227,12 @@ int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order);
 int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags,
                             uint8_t* hd, float* app_out, int* iters_out);
 
-/* Capture plan upload + decode (as ldpc_mixed_decode_async) into a HIP graph owned by p, with
- * an optional host->device copy of llr_host (llr_floats floats into llr) before and a
- * device->host copy of hd (hd_bytes bytes into hd_host) after.  Host buffers must be pinned
- * (hipHostMalloc).  Replays decode whatever plan is current, into the same buffers. */
+/* Build a HIP graph owned by p: an optional host->device copy of llr_host into llr (at most
+ * llr_floats floats), the plan upload, the decode (as ldpc_mixed_decode_async) and an optional
+ * device->host copy of hd into hd_host (at most hd_bytes).  Host buffers must be pinned
+ * (hipHostMalloc).  Each replay decodes the plan current at that time, into the same buffers,
+ * and cuts the copies to the LLR floats and hard-decision bytes that plan addresses.  `stream`
+ * is unused (the graph runs on the replay's stream). */
 int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
                        float* app_out, int* iters_out, const float* llr_host, size_t llr_floats,
                        uint8_t* hd_host, size_t hd_bytes);

@@ -77,6 +77,9 @@ struct MixedArgs {
 };
 
 int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);
+// The same launch as a kernel node of graph g after the given dependencies.
+int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
+                     const hipGraphNode_t* deps, size_t n_deps, hipGraphNode_t* node);
 // Resident workgroups per CU of the mixed kernel at this shape (0 on error).
 int mixed_occupancy(bool learned, int threads, size_t lds);
 

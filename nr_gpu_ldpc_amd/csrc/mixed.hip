@@ -80,6 +80,25 @@ int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t
     return LDPC_OK;
 }
 
+int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
+                     const hipGraphNode_t* deps, size_t n_deps, hipGraphNode_t* node) {
+    if (threads > kMaxBlock || threads < 64 || lds > (size_t)max_lds_bytes() || grid < 1) {
+        set_last_error("mixed launch exceeds the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
+    MixedArgs args = a;  // the node keeps its own copy of the argument values
+    void* params[] = {&args};
+    hipKernelNodeParams kp{};
+    kp.func = learned ? reinterpret_cast<void*>(k_decode_mixed<true>) : reinterpret_cast<void*>(k_decode_mixed<false>);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(threads);
+    kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = params;
+    kp.extra = nullptr;
+    NRLDPC_HIP(hipGraphAddKernelNode(node, g, deps, n_deps, &kp));
+    return LDPC_OK;
+}
+
 int mixed_occupancy(bool learned, int threads, size_t lds) {
     int n = 0;
     hipError_t e = learned ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<true>, threads, lds)

@@ -71,6 +71,14 @@ struct ldpc_mixed {
     bool upload_pending = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // memcpy nodes of the graph (null when absent) and what they copy at most
+    hipGraphNode_t n_llr = nullptr, n_head = nullptr, n_body = nullptr, n_hd = nullptr;
+    const float* g_llr_host = nullptr;
+    float* g_llr = nullptr;
+    uint8_t* g_hd = nullptr;
+    uint8_t* g_hd_host = nullptr;
+    size_t g_llr_floats = 0, g_hd_bytes = 0;
+    long long plan_llr_extent = 0, plan_hd_extent = 0;  // floats / bytes the current plan touches
     std::mutex mu;
 };
 
@@ -202,6 +210,15 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
         const int six[6] = {mt.kind, mt.Z, mt.m_rows, mt.cpb, mt.first, mt.count};
         p->tiles6.insert(p->tiles6.end(), six, six + 6);
     }
+    long long le = 0, he = 0;
+    for (int i = 0; i < n; i++) {
+        const int kb = cws[i].kind == kBG1 ? 22 : 10;
+        const long long nv = (long long)(kb + cws[i].m_rows) * cws[i].Z;
+        le = std::max(le, cws[i].llr_offset + nv);
+        he = std::max(he, cws[i].hd_offset + (nv + 7) / 8);
+    }
+    p->plan_llr_extent = le;
+    p->plan_hd_extent = he;
     *reinterpret_cast<int*>(p->h_plan) = (int)tiles.size();
     p->n_tiles = (int)tiles.size();
     p->n_cw = n;
@@ -210,9 +227,8 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
     return LDPC_OK;
 }
 
-// Enqueue the plan upload (whole buffers when `full`, for graph capture) and the kernel.
-static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
-                         float* app_out, int* iters_out, bool full) {
+static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flags, uint8_t* hd, float* app_out,
+                      int* iters_out, MixedArgs* out) {
     const bool learned = (flags & LDPC_FLAG_LEARNED) != 0;
     if (n_iter < 1) {
         set_last_error("mixed decode: n_iter must be positive");
@@ -222,11 +238,8 @@ static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_i
         set_last_error("mixed decode: learned weights in the plan are for another iteration count");
         return LDPC_EINVAL;
     }
-    const size_t head = full ? p->cws_off : p->tiles_off + (size_t)p->n_tiles * sizeof(MixedTile);
-    const size_t body = (full ? (size_t)p->max_cw : (size_t)p->n_cw) * sizeof(MixedCw);
-    NRLDPC_HIP(hipMemcpyAsync(p->d_plan, p->h_plan, head, hipMemcpyHostToDevice, s));
-    if (body) NRLDPC_HIP(hipMemcpyAsync(p->d_plan + p->cws_off, p->h_plan + p->cws_off, body, hipMemcpyHostToDevice, s));
-    MixedArgs a{};
+    MixedArgs& a = *out;
+    a = MixedArgs{};
     a.tiles = reinterpret_cast<const MixedTile*>(p->d_plan + p->tiles_off);
     a.cws = reinterpret_cast<const MixedCw*>(p->d_plan + p->cws_off);
     a.queue = p->d_queue;
@@ -239,7 +252,23 @@ static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_i
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.fallback = (learned && (flags & LDPC_FLAG_FALLBACK_PLAIN)) ? 1 : 0;
     a.slot_off = p->lds - 16;
-    return mixed_launch(learned, a, p->grid, p->threads, (size_t)p->lds, s);
+    return LDPC_OK;
+}
+
+// Bytes of the plan's two staging regions the current plan uses.
+static size_t plan_head_bytes(const ldpc_mixed* p) { return p->tiles_off + (size_t)p->n_tiles * sizeof(MixedTile); }
+static size_t plan_body_bytes(const ldpc_mixed* p) { return (size_t)std::max(p->n_cw, 1) * sizeof(MixedCw); }
+
+// Enqueue the plan upload and the kernel on stream s.
+static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
+                         float* app_out, int* iters_out) {
+    MixedArgs a;
+    int st = mixed_args(p, llr, n_iter, flags, hd, app_out, iters_out, &a);
+    if (st) return st;
+    NRLDPC_HIP(hipMemcpyAsync(p->d_plan, p->h_plan, plan_head_bytes(p), hipMemcpyHostToDevice, s));
+    NRLDPC_HIP(hipMemcpyAsync(p->d_plan + p->cws_off, p->h_plan + p->cws_off, plan_body_bytes(p),
+                              hipMemcpyHostToDevice, s));
+    return mixed_launch((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, s);
 }
 
 }  // namespace nrldpc
@@ -346,7 +375,7 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
     NRLDPC_HIP(hipSetDevice(p->device));
     if (p->n_tiles == 0) return LDPC_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int st = mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out, false);
+    int st = mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out);
     if (st) return st;
     NRLDPC_HIP(hipEventRecord(p->uploaded, s));
     p->upload_pending = true;
@@ -356,27 +385,44 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
 int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
                        float* app_out, int* iters_out, const float* llr_host, size_t llr_floats, uint8_t* hd_host,
                        size_t hd_bytes) {
-    if (!p || !llr || !hd || !stream) return LDPC_EINVAL;
+    (void)stream;  // the graph is built node by node; it runs on the stream given to the replay
+    if (!p || !llr || !hd) return LDPC_EINVAL;
     if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
     std::lock_guard<std::mutex> lk(p->mu);
     if (!p->planned) { set_last_error("mixed: plan a batch before capturing"); return LDPC_EINVAL; }
     NRLDPC_HIP(hipSetDevice(p->device));
-    hipStream_t s = static_cast<hipStream_t>(stream);
+    MixedArgs a;
+    int st = mixed_args(p, llr, n_iter, flags, hd, app_out, iters_out, &a);
+    if (st) return st;
     if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
     if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
-    NRLDPC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    hipError_t ce = hipSuccess;
-    if (llr_host && llr_floats)
-        ce = hipMemcpyAsync(const_cast<float*>(llr), llr_host, llr_floats * sizeof(float), hipMemcpyHostToDevice, s);
-    int st = ce == hipSuccess ? mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out, true) : LDPC_EHIP;
-    if (st == LDPC_OK && hd_host && hd_bytes)
-        ce = hipMemcpyAsync(hd_host, hd, hd_bytes, hipMemcpyDeviceToHost, s);
+    p->n_llr = p->n_head = p->n_body = p->n_hd = nullptr;
     hipGraph_t g = nullptr;
-    const hipError_t ee = hipStreamEndCapture(s, &g);
-    if (st) { if (g) (void)hipGraphDestroy(g); return st; }
-    if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); NRLDPC_HIP(ce); }
-    NRLDPC_HIP(ee);
+    NRLDPC_HIP(hipGraphCreate(&g, 0));
     p->graph = g;
+    // chain: [LLR upload] -> plan header + tiles -> codeword table -> decode -> [HD download];
+    // every copy is created at its largest size and cut to the current plan at each replay
+    hipGraphNode_t prev = nullptr, kern = nullptr;
+    if (llr_host && llr_floats) {
+        NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_llr, g, nullptr, 0, const_cast<float*>(llr), llr_host,
+                                           llr_floats * sizeof(float), hipMemcpyHostToDevice));
+        prev = p->n_llr;
+    }
+    NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_head, g, prev ? &prev : nullptr, prev ? 1 : 0, p->d_plan, p->h_plan,
+                                       p->cws_off, hipMemcpyHostToDevice));
+    NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_body, g, &p->n_head, 1, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
+                                       (size_t)p->max_cw * sizeof(MixedCw), hipMemcpyHostToDevice));
+    if ((st = mixed_graph_node((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, g,
+                               &p->n_body, 1, &kern)))
+        return st;
+    if (hd_host && hd_bytes)
+        NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_hd, g, &kern, 1, hd_host, hd, hd_bytes, hipMemcpyDeviceToHost));
+    p->g_llr_host = llr_host;
+    p->g_llr = const_cast<float*>(llr);
+    p->g_llr_floats = p->n_llr ? llr_floats : 0;
+    p->g_hd = hd;
+    p->g_hd_host = hd_host;
+    p->g_hd_bytes = p->n_hd ? hd_bytes : 0;
     NRLDPC_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     return LDPC_OK;
 }
@@ -387,6 +433,20 @@ int ldpc_mixed_replay(ldpc_mixed* p, void* stream) {
     if (!p->exec) { set_last_error("mixed: nothing captured"); return LDPC_EINVAL; }
     if (!p->planned) { set_last_error("mixed: no plan"); return LDPC_EINVAL; }
     NRLDPC_HIP(hipSetDevice(p->device));
+    // The previous replay has finished (planning waited for it), so the copies can be resized
+    // to what the current plan reads and writes.
+    const size_t lf = (size_t)std::max<long long>(1, std::min<long long>((long long)p->g_llr_floats, p->plan_llr_extent));
+    const size_t hb = (size_t)std::max<long long>(1, std::min<long long>((long long)p->g_hd_bytes, p->plan_hd_extent));
+    if (p->n_llr)
+        NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_llr, p->g_llr, p->g_llr_host, lf * sizeof(float),
+                                                     hipMemcpyHostToDevice));
+    NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_head, p->d_plan, p->h_plan, plan_head_bytes(p),
+                                                 hipMemcpyHostToDevice));
+    NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_body, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
+                                                 plan_body_bytes(p), hipMemcpyHostToDevice));
+    if (p->n_hd)
+        NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_hd, p->g_hd_host, p->g_hd, hb,
+                                                     hipMemcpyDeviceToHost));
     hipStream_t s = static_cast<hipStream_t>(stream);
     NRLDPC_HIP(hipGraphLaunch(p->exec, s));
     NRLDPC_HIP(hipEventRecord(p->uploaded, s));
@@ -490,7 +550,7 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
         float ms = 0.f;
         for (int rep = 0; rep < 4 && st == LDPC_OK; rep++) {
             if (rep == 1) (void)hipEventRecord(e0, nullptr);
-            st = mixed_enqueue(p, nullptr, llr, n_iter, 0, hd, nullptr, nullptr, false);
+            st = mixed_enqueue(p, nullptr, llr, n_iter, 0, hd, nullptr, nullptr);
         }
         if (st) break;
         (void)hipEventRecord(e1, nullptr);

@@ -93,9 +93,12 @@ def test_mixed_graph_replay_follows_the_plan(nr, torch_gpu):
         llr, _ = mx.make_channel(torch, b, seed=30 + rep)
         torch.cuda.synchronize()
         h_llr[:b["llr_floats"]].copy_(llr.cpu())
+        h_hd.fill_(0xAB)
         m.plan(b["desc"])
         m.replay(s.cuda_stream)
         s.synchronize()
+        # the download is cut to the bytes this plan addresses
+        assert (h_hd.numpy()[b["hd_total"]:] == 0xAB).all(), f"replay {rep}"
         want = _mixed(nr, torch, b, llr, 10, nr.EARLY_EXIT)
         assert np.array_equal(h_hd.numpy()[:b["hd_total"]], want[0][:b["hd_total"]]), f"replay {rep}"
         assert np.array_equal(d_it.cpu().numpy()[:len(b["kind"])], want[2]), f"replay {rep}"
