@@ -320,6 +320,86 @@ def bench_stream(torch, nr, args):
     return out
 
 
+def bench_operating_point(torch, nr, args):
+    """Info throughput at the FER 1e-3 operating point of each decoder (BG1 Z=384, T <= 10 with
+    early exit, batches of 4096): plain min-sum at its FER 1e-3 Eb/N0, the learned decoder (with
+    plain fallback) at its own, as measured for the committed weights."""
+    from nr_gpu_ldpc_amd import noms
+    path = noms.weights_path(nr.BG1, Z, N_ITER)
+    if not os.path.exists(path):
+        return None
+    meta = json.load(open(path.replace(".bin", ".eval.json")))["fer1e-3_db"]
+    code = nr.Code(nr.BG1, Z)
+    code.set_weights(noms.load_weights(path)["weights"])
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    llr = torch.empty((BATCH, code.n_vn), dtype=torch.float32, device=dev)
+    bits = torch.empty((BATCH, code.n_vn), dtype=torch.uint8, device=dev)
+    hd = torch.zeros((BATCH, code.hd_bytes), dtype=torch.uint8, device=dev)
+    its = torch.zeros(BATCH, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(2, dtype=torch.int64, device=dev)
+    out = {"workload": "BG1 Z=384, batch 4096, at most 10 iterations with early exit, at each decoder's "
+                       "FER 1e-3 Eb/N0 (from the committed evaluation)"}
+    steps = max(5, args.steps)
+    wgt = torch.zeros(BATCH, dtype=torch.int32, device=dev)
+    for mode, ebn0 in (("plain", meta["plain"]), ("learned", meta["learned"]),
+                       ("learned_no_fallback", meta["learned"])):
+        flags = nr.EARLY_EXIT | (nr.FALLBACK_PLAIN if mode == "learned" else 0)
+
+        def run(seed):
+            code.channel(seed, 0, BATCH, llr.data_ptr(), sigma=nr.noise_stddev(ebn0, RATE), cw_bits_ptr=bits.data_ptr(),
+                         stream=s)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            if mode.startswith("learned"):
+                code.decode_learned(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), 0, its.data_ptr(), flags=flags, stream=s)
+            else:
+                code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), 0, its.data_ptr(), flags=flags, stream=s)
+            e1.record(stream)
+            code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, cnt.data_ptr(), stream=s)
+            return e0, e1
+
+        run(1)
+        cnt.zero_()
+        torch.cuda.synchronize()
+        evs, iters, sfail = [], 0, 0
+        for k in range(steps):
+            evs.append(run(100 + k))
+            code.syndrome(hd.data_ptr(), BATCH, wgt.data_ptr(), stream=s)
+            iters += int(its.sum())
+            sfail += int(torch.count_nonzero(wgt))
+        torch.cuda.synchronize()
+        per = [a.elapsed_time(b) for a, b in evs]
+        be, fe = cnt.tolist()
+        out[mode] = {"ebn0_db": ebn0, "info_gbps": round(BATCH * code.K / (statistics.mean(per) * 1e6), 3),
+                     "latency_ms": {"p50": round(statistics.median(per), 3), "p99": round(pct(per, 0.99), 3)},
+                     "mean_iterations": round(iters / (steps * BATCH), 3),
+                     "fer_measured": fe / (steps * BATCH), "syndrome_fail": sfail / (steps * BATCH),
+                     "frames": steps * BATCH}
+    return out
+
+
+def bench_reference_example():
+    """The reference's shipped example through the BER/FER driver (12 x 24 code, Z=96, T=5,
+    80 codewords x 8 streams per SNR point, SNR 0..10 dB; host-generated data, PCIe included)."""
+    import subprocess
+    exe = os.path.join(ROOT, "nr_gpu_ldpc_amd", "bin", "ldpc_ber")
+    if not os.path.exists(exe):
+        return None
+    args = ["--code", "qc", "--Z", "96", "--iters", "5", "--snr-min", "0", "--snr-max", "10", "--snr-step", "1",
+            "--codewords", "80", "--cw-per-mcw", "8", "--mcw", "10", "--streams", "8", "--seed", "69012"]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    lines = [json.loads(x) for x in r.stdout.splitlines()]
+    return {"workload": "12 x 24 QC code, Z=96 (K=1152, N=2304), flooding min-sum T=5, 640 frames per SNR point "
+                        "(80 unique x 8 streams), host rand/drand48 data as the reference generates it",
+            "command": "nr_gpu_ldpc_amd/bin/ldpc_ber " + " ".join(args),
+            "points": [{k: x[k] for k in ("snr_db", "ber", "fer", "best_latency_us", "user_throughput_mbps")}
+                       for x in lines]}
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -356,10 +436,17 @@ def main():
         "ber": head["ber"],
         "roofline": head["roofline"],
     }
+    op = bench_operating_point(torch, nr, args)
+    if op:
+        out["fer1e-3_operating_point"] = op
     if not args.skip_mixed:
         out["mixed_batch"] = bench_mixed(torch, nr, args)
     if not args.skip_stream:
         out["combined_stream"] = bench_stream(torch, nr, args)
+    if not args.skip_stream:
+        ex = bench_reference_example()
+        if ex:
+            out["reference_example"] = ex
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(llr.cpu().numpy(), min(args.cpu_sample, BATCH))
     print(json.dumps(out, default=lambda o: o.item() if hasattr(o, "item") else str(o)))
