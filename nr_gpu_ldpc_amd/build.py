@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Build recipe for libnrldpc.so (HIP for gfx950) and the C++ driver executable.
 
-    python -m nr_gpu_ldpc_amd.build        # or: python nr_gpu_ldpc_amd/build.py
+    python nr_gpu_ldpc_amd/build.py        # (a script: importing the package needs the library)
 
 Every source under csrc/ is compiled with hipcc for gfx950 into _build/, then linked into
 nr_gpu_ldpc_amd/lib/libnrldpc.so (kept in-tree so it travels with the repository snapshot).
