@@ -153,12 +153,26 @@ def bench_headline(torch, nr, args):
     valu_gops = valu_ops / (ms * 1e6)
     hbm_gbs = alg_bytes / (ms * 1e6)
     traffic, tsrc = traffic_per_launch()
+    # the reference's HALF_PRC schedule on the same batch (binary16 LLRs, 2 bytes per LLR)
+    llr_h = torch.empty((BATCH, code.n_vn), dtype=torch.int16, device=dev)
+    nr.llr_to_half(llr.data_ptr(), llr.numel(), llr_h.data_ptr(), stream=s)
+    code.decode_half(llr_h.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    hev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    hev[0].record(stream)
+    for _ in range(args.steps):
+        code.decode_half(llr_h.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+    hev[1].record(stream)
+    torch.cuda.synchronize()
+    half_ms = hev[0].elapsed_time(hev[1]) / args.steps
     res = {
         "value": round(gbps, 4), "ms_per_step": round(ms, 4),
         "cw_per_block": code.cw_per_block,
         "latency_ms": {"p50": round(statistics.median(per), 4), "p99": round(pct(per, 0.99), 4),
                        "kind": "device time per batch of 4096"},
         "fer": round(frame_err / BATCH, 6), "ber": bit_err / (BATCH * code.K),
+        "half_prc": {"info_gbps": round(BATCH * code.K / (half_ms * 1e6), 4), "ms_per_step": round(half_ms, 4),
+                     "note": "the reference's HALF_PRC schedule: binary16 LLRs, R, dt and inter-iteration APP"},
         "roofline": {
             "bound": "hbm",
             "achieved": round(hbm_gbs, 1),
@@ -435,6 +449,7 @@ def main():
         "fer": head["fer"],
         "ber": head["ber"],
         "roofline": head["roofline"],
+        "half_prc": head["half_prc"],
     }
     op = bench_operating_point(torch, nr, args)
     if op:

@@ -81,6 +81,17 @@ int ldpc_ctx_info(const ldpc_ctx* ctx, ldpc_code_info* info);
 int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
                       unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 
+/* The reference's HALF_PRC build: binary16 channel LLRs (IEEE bit patterns, [n_cw][n_vn]) and
+ * R, dt and the posterior between iterations stored in binary16 where the reference stores them
+ * (src/ldpc_kernels.cu HALF_PRC branches: :158-160, :258-266, :304-306, :351, :385-386), all
+ * arithmetic in single precision; outputs as for ldpc_decode_async (app_out: the final
+ * single-precision posterior). */
+int ldpc_decode_half_async(ldpc_ctx* ctx, void* stream, const uint16_t* llr_half, int n_cw, int n_iter,
+                           unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+/* float LLRs -> binary16 bit patterns (round to nearest even), n values; the reference's
+ * reduce_precision<half>, src/setup_exp.cu:33-38, on the device. */
+int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out);
+
 /* Codewords per workgroup of the decode launch (0 restores the context's default).
  * No reference counterpart: the reference fixes n_cw_per_mcw at build time
  * (setup_gpu_grid src/setup_exp.cu:131-166). */

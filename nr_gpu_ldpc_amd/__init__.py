@@ -25,7 +25,7 @@ LEARNED = 4
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"{LIB_PATH} is missing: build the HIP library first (python -m nr_gpu_ldpc_amd.build)")
+        f"{LIB_PATH} is missing: build the HIP library first (python nr_gpu_ldpc_amd/build.py)")
 
 
 def _single_hip_runtime():
@@ -79,6 +79,8 @@ _sig("ldpc_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, 
 _sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_decode_half_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_llr_to_half_async", ctypes.c_int, _P, _P, ctypes.c_size_t, _P)
 _sig("ldpc_base_matrix", ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_int)
 _sig("ldpc_setup_params", ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_int_p)
 _sig("ldpc_encode_host", ctypes.c_int, _P, _P, ctypes.c_int, _P)
@@ -123,6 +125,11 @@ class LdpcError(RuntimeError):
 def _check(status, what):
     if status != 0:
         raise LdpcError(status, what)
+
+
+def llr_to_half(llr_ptr, n, out_ptr, stream=0):
+    """Device float32 -> binary16 (round to nearest even), n values."""
+    _check(lib.ldpc_llr_to_half_async(stream, llr_ptr, n, out_ptr), "ldpc_llr_to_half_async")
 
 
 def device_count():
@@ -202,6 +209,11 @@ class Code:
     def decode(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
         _check(lib.ldpc_decode_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,
                                      app_ptr or None, iters_ptr or None), "ldpc_decode_async")
+
+    def decode_half(self, llr_half_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        """The reference's HALF_PRC schedule; llr_half_ptr: device binary16 [n_cw][n_vn]."""
+        _check(lib.ldpc_decode_half_async(self._h, stream, llr_half_ptr, n_cw, n_iter, flags, hd_ptr,
+                                          app_ptr or None, iters_ptr or None), "ldpc_decode_half_async")
 
     def syndrome(self, hd_ptr, n_cw, weight_ptr, stream=0):
         _check(lib.ldpc_syndrome_async(self._h, stream, hd_ptr, n_cw, weight_ptr), "ldpc_syndrome_async")

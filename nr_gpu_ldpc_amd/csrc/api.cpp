@@ -385,6 +385,18 @@ int ldpc_count_errors_async(ldpc_ctx* x, void* stream, const uint8_t* hd, const 
     return count_errors_launch(a, n_cw, static_cast<hipStream_t>(stream));
 }
 
+int ldpc_decode_half_async(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_iter, unsigned flags,
+                           uint8_t* hd, float* app_out, int* iters_out) {
+    if (!x || !llr || !hd || n_cw < 0 || n_iter < 1) return LDPC_EINVAL;
+    if (n_cw == 0) return LDPC_OK;
+    return decode_half(x, stream, llr, n_cw, n_iter, flags, hd, app_out, iters_out);
+}
+
+int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out) {
+    if ((!llr || !out) && n) return LDPC_EINVAL;
+    return to_half_launch(llr, n, out, static_cast<hipStream_t>(stream));
+}
+
 int ldpc_hd_to_stream_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, uint8_t* out) {
     if (!x || !hd || !out || n_cw < 0) return LDPC_EINVAL;
     if (n_cw == 0) return LDPC_OK;

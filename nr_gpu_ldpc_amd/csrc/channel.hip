@@ -7,6 +7,7 @@
 // reproducible bit for bit on any IEEE-754 host.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "channel.h"
@@ -241,6 +242,19 @@ int stream_pack_launch(const StreamPackArgs& a, hipStream_t s) {
         return LDPC_EINVAL;
     }
     hipLaunchKernelGGL(k_stream_pack, dim3((unsigned)blocks), dim3(256), 0, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
+__global__ void __launch_bounds__(256) k_to_half(const float* in, size_t n, uint16_t* out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = __builtin_bit_cast(uint16_t, (_Float16)in[i]);
+}
+
+int to_half_launch(const float* in, size_t n, uint16_t* out, hipStream_t s) {
+    if (n == 0) return LDPC_OK;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_to_half, dim3((unsigned)blocks), dim3(256), 0, s, in, n, out);
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;
 }

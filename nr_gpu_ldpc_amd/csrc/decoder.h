@@ -37,6 +37,7 @@ struct DecodeArgs {
     float* rec_m2;
     uint32_t* rec_w;       // Q sign bits (edge k at bit d-1-k) | arg-min << 20 | 2nd arg-min << 25
     const int* select;     // [n_cw] or null: decode only codewords with select[cw] != 0
+    const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -86,6 +87,8 @@ int mixed_occupancy(bool learned, int threads, size_t lds);
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
 int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
+// The reference's HALF_PRC schedule (binary16 LLRs in a.llr_half).
+int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s);
 // Learned forward that also writes the training records (generic-Z kernel).
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s);
 

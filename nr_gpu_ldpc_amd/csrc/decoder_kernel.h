@@ -86,6 +86,10 @@ __device__ __forceinline__ bool block_any(bool v, uint32_t slot) {
     return any;
 }
 
+// binary16 storage of the HALF_PRC schedule: round to nearest even and back.
+__device__ __forceinline__ float round_half(float x) { return (float)(_Float16)x; }
+__device__ __forceinline__ float half_bits_to_float(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
+
 // Per-edge table entries are read through the constant address space so that the compiler
 // issues scalar (s_load) reads of these wave-uniform values.
 using CEdge = const EdgeParam __attribute__((address_space(4)));
@@ -160,7 +164,10 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // syndrome after the last iteration; `emit_only_ok` then writes outputs only for codewords
 // whose hard decision passes it.  Returns, per lane, whether its codeword converged (early
 // exit) or passed the final check.
-template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED>
+// HALF: the reference's HALF_PRC schedule -- binary16 channel LLRs (a.llr_half), and R, dt and
+// the APP between iterations rounded to binary16 where the reference stores them, all
+// arithmetic in single precision.
+template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false>
 __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                                              const MixedCw* __restrict__ mcw, const bool sel = true,
                                              const bool final_check = false,
@@ -168,6 +175,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     using S = Structure<KIND>;
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && (TRAIN || ZC > 0)), "mixed tiles use the any-Z plain/learned path");
+    static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -218,10 +226,15 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
 
     // ---- load channel LLRs
     const float* __restrict__ L = MIXED ? a.llr + mcw[cwc].llr_off : a.llr + (size_t)cwc * a.n_vn;
+    const uint16_t* __restrict__ Lh = HALF ? a.llr_half + (size_t)cwc * a.n_vn : nullptr;
+    auto llr_at = [&](int n) -> float {
+        if constexpr (HALF) return half_bits_to_float(Lh[n]);
+        else return L[n];
+    };
     if (!idle) {
 #pragma unroll 2
         for (int j = 0; j < NND; j++) {
-            const float v = L[j * Z + r] + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
+            const float v = llr_at(j * Z + r) + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
             lds_st(rRd + j * 2 * Z4, v);
             lds_st(rRd + j * 2 * Z4 + Z4, v);
             lds_st(rAcc + j * Z4, v);
@@ -236,7 +249,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             if constexpr (ZC > 0) c = r + edge_vm<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = L[j * Z + c] + 0.0f;
+            if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = llr_at(j * Z + c) + 0.0f;
             else app1[S::T.d1_slot[j]] = 0.0f;
         }
     });
@@ -405,10 +418,16 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     }
                     const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
                     if constexpr (MIXED) dtv[k] = row_on ? rn - ro[k] : 0.0f;
+                    else if constexpr (HALF) dtv[k] = round_half(rn - ro[k]);
                     else dtv[k] = rn - ro[k];
                 });
-                A1[i] = n1;
-                A2[i] = n2;
+                if constexpr (HALF) {  // R is stored in binary16; dt used the unrounded R_new
+                    A1[i] = round_half(n1);
+                    A2[i] = round_half(n2);
+                } else {
+                    A1[i] = n1;
+                    A2[i] = n2;
+                }
                 constexpr uint32_t dmask = (1u << D) - 1u;
                 const uint32_t fresh_w = ((sgn ? ~neg : neg) & dmask) | (idxn << (IO - WO));
                 if constexpr (!S::half(i)) W[WW] = fresh_w;
@@ -472,9 +491,17 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         if (!idle) {
 #pragma unroll 2
             for (int j = 0; j < NND; j++) {
-                const float v = lds_ld(rAcc + j * Z4);
+                float v = lds_ld(rAcc + j * Z4);
+                if constexpr (HALF) {  // the APP between iterations is stored in binary16
+                    v = round_half(v);
+                    lds_st(rAcc + j * Z4, v);
+                }
                 lds_st(rRd + j * 2 * Z4, v);
                 lds_st(rRd + j * 2 * Z4 + Z4, v);
+            }
+            if constexpr (HALF) {
+#pragma unroll
+                for (int n = 0; n < ND1; n++) app1[n] = round_half(app1[n]);
             }
         }
         const bool check = early || fin;
@@ -525,9 +552,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     return emitted || final_ok;
 }
 
-template <int KIND, int ZC, bool LEARNED, bool TRAIN = false>
+template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
-    decode_block<KIND, ZC, LEARNED, TRAIN, false>(a, blockIdx.x, nullptr);
+    decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF>(a, blockIdx.x, nullptr);
 }
 
 }  // namespace nrldpc

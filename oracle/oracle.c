@@ -204,9 +204,57 @@ int oracle_syndrome(const oracle_code *c, const uint8_t *hd) {
 
 /* One codeword of (learned) scaled min-sum in single precision.  alpha/beta == NULL selects
  * the reference constant 0.75f and no offset, computed exactly as the reference writes it. */
+/* IEEE binary16 conversions (round to nearest, ties to even), written out from the format
+ * definition: what __float2half / __half2float do for the reference's HALF_PRC build. */
+uint16_t oracle_float_to_half(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    const uint32_t sign = (x >> 16) & 0x8000u;
+    const uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) /* inf, nan (quiet, payload kept where it fits) */
+        return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u | ((ax >> 13) & 0x3ffu) : 0u));
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520: rounds to inf */
+    if (ax >= 0x38800000u) {                                  /* normal half */
+        uint32_t m = ax + 0xfffu + ((ax >> 13) & 1u);
+        m -= (uint32_t)(127 - 15) << 23;
+        return (uint16_t)(sign | (m >> 13));
+    }
+    if (ax < 0x33000000u) return (uint16_t)sign; /* below half the smallest subnormal */
+    const uint32_t e = ax >> 23;
+    const uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+    const uint32_t shift = 126u - e; /* value = q * 2^-24 */
+    uint32_t q = mant >> shift;
+    const uint32_t rem = mant & ((1u << shift) - 1u), halfway = 1u << (shift - 1u);
+    if (rem > halfway || (rem == halfway && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+}
+
+float oracle_half_to_float(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+    uint32_t x;
+    if (e == 31) x = sign | 0x7f800000u | (m << 13);
+    else if (e != 0) x = sign | ((e + 112u) << 23) | (m << 13);
+    else if (m == 0) x = sign;
+    else { /* subnormal: normalise */
+        int k = 0;
+        uint32_t mm = m;
+        while (!(mm & 0x400u)) { mm <<= 1; k++; }
+        x = sign | ((uint32_t)(113 - k) << 23) | ((mm & 0x3ffu) << 13);
+    }
+    float f;
+    memcpy(&f, &x, 4);
+    return f;
+}
+
+static inline float rh(float x) { return oracle_half_to_float(oracle_float_to_half(x)); }
+
+/* half_store: the HALF_PRC build, where R, dt and the APP between iterations are stored as
+ * binary16 (src/ldpc_kernels.cu:158-160,266,304-306,351,385-386) and all arithmetic is single
+ * precision; the last iteration's APP is not stored (:450-467). */
 static void decode_one(const oracle_code *c, const edge_list *el, const float *llr, int n_iter,
                        int early_exit, const float *alpha, const float *beta, float *R, float *dt,
-                       float *app, uint8_t *hd, int *iters_out) {
+                       float *app, uint8_t *hd, int *iters_out, int half_store) {
     const int Z = c->Z, nvn = c->cols * Z;
     memcpy(app, llr, sizeof(float) * nvn);
     memset(R, 0, sizeof(float) * el->n_edges * Z);
@@ -249,8 +297,9 @@ static void decode_one(const oracle_code *c, const edge_list *el, const float *l
                         Rn = sign * (float)sq * (alpha[b] * off);
                     }
                     const float Ro = (it == 0) ? 0.0f : R[e * Z + r];
-                    dt[e * Z + r] = (it == 0) ? Rn : Rn - Ro; /* :177, :308-310 */
-                    R[e * Z + r] = Rn;
+                    const float d = (it == 0) ? Rn : Rn - Ro; /* :177, :308-310 */
+                    dt[e * Z + r] = half_store ? rh(d) : d;
+                    R[e * Z + r] = half_store ? rh(Rn) : Rn;
                 }
             }
         }
@@ -265,7 +314,7 @@ static void decode_one(const oracle_code *c, const edge_list *el, const float *l
                     if (sf < 0) sf += Z;
                     a += dt[e * Z + sf];
                 }
-                app[j * Z + s] = a;
+                app[j * Z + s] = (half_store && it + 1 < n_iter) ? rh(a) : a;
             }
         }
         /* Hard decision: APP > 0 -> 0, else 1 (src/ldpc_kernels.cu:463-467). */
@@ -284,6 +333,7 @@ typedef struct {
     uint8_t *hd;
     float *app;
     int *iters;
+    int half_store;
 } dec_job;
 
 static void *dec_worker(void *arg) {
@@ -297,7 +347,8 @@ static void *dec_worker(void *arg) {
     const int b = jb->tid * per, e = (b + per < jb->n_cw) ? b + per : jb->n_cw;
     for (int w = b; w < e; w++) {
         decode_one(jb->c, jb->el, jb->llr + (size_t)w * nvn, jb->n_iter, jb->early_exit, jb->alpha,
-                   jb->beta, R, dt, app, jb->hd + (size_t)w * nvn, jb->iters ? jb->iters + w : NULL);
+                   jb->beta, R, dt, app, jb->hd + (size_t)w * nvn, jb->iters ? jb->iters + w : NULL,
+                   jb->half_store);
         if (jb->app) memcpy(jb->app + (size_t)w * nvn, app, sizeof(float) * nvn);
     }
     free(R); free(dt); free(app);
@@ -306,7 +357,7 @@ static void *dec_worker(void *arg) {
 
 static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
                         const float *alpha, const float *beta, uint8_t *hd, float *app, int *iters,
-                        int threads) {
+                        int threads, int half_store) {
     edge_list el;
     edges_build(c, &el);
     if (threads < 1) threads = 1;
@@ -315,7 +366,7 @@ static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_
     dec_job jobs[256];
     if (threads > 256) threads = 256;
     for (int t = 0; t < threads; t++) {
-        dec_job j = {c, &el, llr, n_cw, n_iter, early_exit, t, threads, alpha, beta, hd, app, iters};
+        dec_job j = {c, &el, llr, n_cw, n_iter, early_exit, t, threads, alpha, beta, hd, app, iters, half_store};
         jobs[t] = j;
         if (threads == 1) dec_worker(&jobs[t]);
         else pthread_create(&th[t], NULL, dec_worker, &jobs[t]);
@@ -327,12 +378,21 @@ static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_
 
 void oracle_decode(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
                    uint8_t *hd, float *app, int *iters, int threads) {
-    decode_many(c, llr, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads);
+    decode_many(c, llr, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, 0);
+}
+
+void oracle_decode_half(const oracle_code *c, const uint16_t *llr, int n_cw, int n_iter, int early_exit,
+                        uint8_t *hd, float *app, int *iters, int threads) {
+    const size_t n = (size_t)n_cw * c->cols * c->Z;
+    float *f = malloc(sizeof(float) * (n ? n : 1));
+    for (size_t i = 0; i < n; i++) f[i] = oracle_half_to_float(llr[i]);
+    decode_many(c, f, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, 1);
+    free(f);
 }
 
 void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter,
                         const float *alpha, const float *beta, uint8_t *hd, float *app) {
-    decode_many(c, llr, n_cw, n_iter, 0, alpha, beta, hd, app, NULL, 1);
+    decode_many(c, llr, n_cw, n_iter, 0, alpha, beta, hd, app, NULL, 1, 0);
 }
 
 /* ---------------------------------------------------------------- NOMS in double + gradient */

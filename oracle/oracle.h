@@ -48,6 +48,13 @@ void oracle_setup_ldpc(int A, float rate, int *out);
 void oracle_decode(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
                    uint8_t *hd, float *app, int *iters, int threads);
 
+/* The reference's HALF_PRC build: LLRs, R, dt and the APP between iterations stored as IEEE
+ * binary16, arithmetic in single precision (src/ldpc_kernels.cu, HALF_PRC branches). */
+void oracle_decode_half(const oracle_code *c, const uint16_t *llr, int n_cw, int n_iter, int early_exit,
+                        uint8_t *hd, float *app, int *iters, int threads);
+uint16_t oracle_float_to_half(float f);  /* round to nearest even */
+float oracle_half_to_float(uint16_t h);
+
 /* Learned offset/normalized min-sum: R = alpha[t][e] * sign * max(m - beta[t][e], 0) with one
  * (alpha, beta) per (iteration, base-graph edge); alpha = 0.75, beta = 0 is the plain decoder
  * above.  Single precision, same schedule as oracle_decode. */

@@ -29,6 +29,12 @@ class Oracle:
         lib.oracle_decode.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       P, P, P, ctypes.c_int]
         lib.oracle_decode_noms.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, P, P, P, P]
+        lib.oracle_decode_half.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           P, P, P, ctypes.c_int]
+        lib.oracle_float_to_half.argtypes = [ctypes.c_float]
+        lib.oracle_float_to_half.restype = ctypes.c_uint16
+        lib.oracle_half_to_float.argtypes = [ctypes.c_uint16]
+        lib.oracle_half_to_float.restype = ctypes.c_float
         lib.oracle_noms_loss_grad.argtypes = [ctypes.POINTER(OracleCode), P, P, ctypes.c_int, ctypes.c_int,
                                               P, P, P, P]
         lib.oracle_noms_loss_grad.restype = ctypes.c_double
@@ -64,6 +70,17 @@ class Oracle:
         iters = np.empty(n_cw, dtype=np.int32)
         self.lib.oracle_decode(ctypes.byref(c), llr.ctypes.data, n_cw, n_iter, int(early_exit), hd.ctypes.data,
                                app.ctypes.data, iters.ctypes.data, threads)
+        return hd, app, iters
+
+    def decode_half(self, c, llr_half, n_iter, early_exit=False, threads=8):
+        """llr_half: uint16 (IEEE binary16 bit patterns) [n_cw][cols*Z]."""
+        llr_half = np.ascontiguousarray(llr_half, dtype=np.uint16)
+        n_cw, nvn = llr_half.shape
+        hd = np.empty((n_cw, nvn), dtype=np.uint8)
+        app = np.empty((n_cw, nvn), dtype=np.float32)
+        iters = np.empty(n_cw, dtype=np.int32)
+        self.lib.oracle_decode_half(ctypes.byref(c), llr_half.ctypes.data, n_cw, n_iter, int(early_exit),
+                                    hd.ctypes.data, app.ctypes.data, iters.ctypes.data, threads)
         return hd, app, iters
 
     def decode_noms(self, c, llr, n_iter, alpha, beta):

@@ -59,9 +59,11 @@ def _dense_H(oracle, code):
     return H
 
 
-def dense_minsum(H, llr, n_iter):
-    """Flooding scaled min-sum on a dense H, written from the reference kernels' arithmetic."""
+def dense_minsum(H, llr, n_iter, half=False):
+    """Flooding scaled min-sum on a dense H, written from the reference kernels' arithmetic.
+    half: the HALF_PRC build (R, dt and the APP between iterations rounded to binary16)."""
     M, N = H.shape
+    st = (lambda v: np.asarray(v, F).astype(np.float16).astype(F)) if half else (lambda v: np.asarray(v, F))
     app = llr.astype(F).copy()
     R = np.zeros((M, N), dtype=F)
     nbr = [np.nonzero(H[m])[0] for m in range(M)]
@@ -79,11 +81,13 @@ def dense_minsum(H, llr, n_iter):
             sign = F(np.prod(sq))
             mag = np.where(np.arange(len(cols)) == idx, m2, m1).astype(F)
             Rn = ((F(0.75) * sign) * sq * mag).astype(F)
-            dt[m, cols] = Rn if t == 0 else (Rn - R[m, cols]).astype(F)
-            R[m, cols] = Rn
+            dt[m, cols] = st(Rn if t == 0 else (Rn - R[m, cols]).astype(F))
+            R[m, cols] = st(Rn)
         for m in range(M):  # ascending block row per column
             cols = nbr[m]
             app[cols] = (app[cols] + dt[m, cols]).astype(F)
+        if t + 1 < n_iter:
+            app = st(app)
     return (app <= 0).astype(np.uint8), app
 
 
@@ -223,3 +227,40 @@ def test_llr_closed_form(oracle):
     out2 = np.full(4, np.nan, dtype=F)
     oracle.lib.oracle_llr(recv.ctypes.data, 4, ctypes.c_float(0.5), 0, out2.ctypes.data)
     assert out2.tolist() == [4.0, -10.0, 16.0, 0.0]  # sigma = 0.5: LLR = 8 y exactly
+
+
+def test_half_conversions_match_ieee(oracle):
+    """binary16 rounding (ties to even, subnormals, overflow) against numpy's IEEE conversion."""
+    rng = np.random.default_rng(0)
+    v = (rng.standard_normal(4000) * 10.0 ** rng.integers(-9, 6, 4000)).astype(F)
+    v = np.concatenate([v, np.array([0.0, -0.0, 65504, 65519.99, 65520, 1e-8, 2 ** -24, 2 ** -25, 3 * 2 ** -26,
+                                     6.1e-5, np.inf, -np.inf], F)])
+    with np.errstate(over="ignore"):
+        want = v.astype(np.float16).view(np.uint16)
+    got = np.array([oracle.lib.oracle_float_to_half(float(x)) for x in v], np.uint16)
+    assert np.array_equal(got, want)
+    h = np.arange(0, 65536, 97, dtype=np.uint16)
+    back = np.array([oracle.lib.oracle_half_to_float(int(x)) for x in h], F)
+    ref = h.view(np.float16).astype(F)
+    assert np.array_equal(back.view(np.uint32)[~np.isnan(ref)], ref.view(np.uint32)[~np.isnan(ref)])
+
+
+@pytest.mark.parametrize("kind,Z,m", [(OR_WIFI, 5, 0), (OR_BG2, 2, 0), (OR_BG1, 3, 0)])
+def test_oracle_half_matches_dense_restatement(oracle, kind, Z, m):
+    """HALF_PRC: the oracle equals the dense restatement with binary16 storage points."""
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    rng = np.random.default_rng(Z * 13 + kind)
+    n = H.shape[1]
+    llr = (rng.standard_normal((3, n)) * 2.5 + 1.0).astype(np.float16)
+    if kind != OR_WIFI:
+        llr[:, :2 * Z] = 0
+    llr[2] = np.round(llr[2])
+    hd, app, _ = oracle.decode_half(code, llr.view(np.uint16), 5)
+    for w in range(3):
+        dhd, dapp = dense_minsum(H, llr[w].astype(F), 5, half=True)
+        assert np.array_equal(hd[w], dhd)
+        assert np.array_equal(app[w], dapp)
+    # and it differs from the single-precision decoder somewhere (the storage points matter)
+    f_hd, f_app, _ = oracle.decode(code, llr.astype(F), 5)
+    assert not np.array_equal(f_app, app)
