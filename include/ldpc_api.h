@@ -266,6 +266,50 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
 int ldpc_mixed_save_tuning(const ldpc_mixed* p, const char* path);
 int ldpc_mixed_load_tuning(ldpc_mixed* p, const char* path);
 
+/* ----------------------------------------------------------------- transport blocks
+ * TS 38.212 around the decoder: CRC (5.1), code-block segmentation with C > 1 (5.2.2), filler
+ * bits (5.3.2), rate matching for LDPC (5.4.2: bit selection from the full circular buffer,
+ * I_LBRM = 0, one layer, and bit interleaving).  No reference counterpart: the reference
+ * handles one code block (setup_ldpc rejects C > 1, src/nr_ldpc.cpp:78-81) and no rate
+ * matching. */
+#define LDPC_CRC16 0   /* gCRC16 */
+#define LDPC_CRC24A 1  /* gCRC24A */
+#define LDPC_CRC24B 2  /* gCRC24B */
+
+/* CRC parity of bits[0..n) (one bit per byte, bit 0 first), register initialised to 0: the
+ * parity bits p_0..p_{L-1} are bits L-1..0 of *crc. */
+int ldpc_crc(int type, const uint8_t* bits, long long n, uint32_t* crc);
+
+typedef struct ldpc_tb ldpc_tb;
+typedef struct {
+    int A, L_tb, B;     /* payload bits, TB CRC bits (16 or 24), A + L_tb */
+    int bg, C, L_cb;    /* base graph, code blocks, code-block CRC bits (0 or 24) */
+    int K_prime, K, Z;  /* bits per block before filler (incl. CRC), kb*Z, lifting size */
+    int F, N, n_vn;     /* filler bits per block, circular buffer (cols-2)*Z, decoder columns */
+    int G, Qm, rv, k0;  /* rate matching: coded bits, modulation order, redundancy version, k0 */
+} ldpc_tb_info;
+
+/* Configure a transport block of A bits at nominal rate `rate` (base graph per TS 38.212 7.2.2
+ * when bg = 0), rate-matched to G bits (multiple of Qm) with redundancy version rv.
+ * device < 0: host only (encoding); device >= 0 also uploads the de-rate-matching map. */
+int ldpc_tb_create(ldpc_tb** out, int A, float rate, int bg, int G, int Qm, int rv, int device);
+int ldpc_tb_destroy(ldpc_tb* tb);
+/* Parameters, and the rate-matched length E_r of code blocks r < max_e into E (nullable). */
+int ldpc_tb_get_info(const ldpc_tb* tb, ldpc_tb_info* info, int* E, int max_e);
+/* Host: payload a[A] (one bit per byte) -> transmitted bits f[G] (CRC, segmentation, LDPC
+ * encoding, rate matching, interleaving); cw[C][n_vn] (nullable) receives the code blocks. */
+int ldpc_tb_encode_host(const ldpc_tb* tb, const uint8_t* a, uint8_t* f, uint8_t* cw);
+/* Device: received LLRs llr_in[G] (in transmitted order) -> decoder LLRs llr_cb[C][n_vn]:
+ * repetitions summed in bit-selection order, filler bits +10000 (known 0), punctured and
+ * unsent columns 0.  Decode llr_cb with a context of ldpc_tb_decoder_context. */
+int ldpc_tb_derate_async(const ldpc_tb* tb, void* stream, const float* llr_in, float* llr_cb);
+/* Device: packed decisions hd[C][ceil(n_vn/8)] -> cb_ok[C] (CRC24B of each block; 1 when
+ * C = 1) and, if tb_bits is not NULL, the B transport-block bits (payload + TB CRC, one
+ * per byte); check the TB CRC with ldpc_crc over them (remainder 0). */
+int ldpc_tb_check_async(const ldpc_tb* tb, void* stream, const uint8_t* hd, int* cb_ok, uint8_t* tb_bits);
+/* A decoder context for the transport block's code (bg, Z, all rows) on `device`. */
+int ldpc_tb_decoder_context(const ldpc_tb* tb, int device, ldpc_ctx** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,7 +10,7 @@ torch tensor on ``cuda:0``) and streams as integer hipStream_t handles (0 = defa
 import ctypes
 import os
 
-__all__ = ["lib", "Code", "Mixed", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "FALLBACK_PLAIN",
+__all__ = ["lib", "Code", "Mixed", "TransportBlock", "crc", "LdpcError", "BG1", "BG2", "QC12X24", "EARLY_EXIT", "FALLBACK_PLAIN",
            "LEARNED", "MIXED_CW_DTYPE", "setup_params", "base_matrix", "noise_stddev", "generator_shifts",
            "device_count"]
 
@@ -22,6 +22,7 @@ BG1, BG2, QC12X24 = 1, 2, 3
 EARLY_EXIT = 1
 FALLBACK_PLAIN = 2
 LEARNED = 4
+CRC16, CRC24A, CRC24B = 0, 1, 2
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -112,6 +113,14 @@ _sig("ldpc_mixed_autotune", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes
      ctypes.c_int, _c_int_p, _c_int_p)
 _sig("ldpc_mixed_save_tuning", ctypes.c_int, _P, ctypes.c_char_p)
 _sig("ldpc_mixed_load_tuning", ctypes.c_int, _P, ctypes.c_char_p)
+_sig("ldpc_crc", ctypes.c_int, ctypes.c_int, _P, ctypes.c_longlong, ctypes.POINTER(ctypes.c_uint32))
+_sig("ldpc_tb_create", ctypes.c_int, ctypes.POINTER(_P), ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+     ctypes.c_int, ctypes.c_int, ctypes.c_int)
+_sig("ldpc_tb_destroy", ctypes.c_int, _P)
+_sig("ldpc_tb_get_info", ctypes.c_int, _P, _P, _c_int_p, ctypes.c_int)
+_sig("ldpc_tb_encode_host", ctypes.c_int, _P, _P, _P, _P)
+_sig("ldpc_tb_derate_async", ctypes.c_int, _P, _P, _P, _P)
+_sig("ldpc_tb_check_async", ctypes.c_int, _P, _P, _P, _P, _P)
 
 
 class LdpcError(RuntimeError):
@@ -374,3 +383,62 @@ class Mixed:
 
     def load_tuning(self, path):
         _check(lib.ldpc_mixed_load_tuning(self._h, os.fsencode(path)), "ldpc_mixed_load_tuning")
+
+
+def crc(kind, bits):
+    """CRC parity (TS 38.212 5.1) of a 0/1 bit sequence, as an int (p_0 = most significant)."""
+    import numpy as np
+    b = np.ascontiguousarray(bits, dtype=np.uint8)
+    out = ctypes.c_uint32()
+    _check(lib.ldpc_crc(kind, b.ctypes.data, b.size, ctypes.byref(out)), "ldpc_crc")
+    return out.value
+
+
+class TbInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("A", "L_tb", "B", "bg", "C", "L_cb", "K_prime", "K", "Z", "F", "N",
+                                              "n_vn", "G", "Qm", "rv", "k0")]
+
+
+class TransportBlock:
+    """A transport block of A bits, segmented, LDPC-coded and rate-matched to G bits (TS 38.212).
+    device < 0: host only (encoding)."""
+
+    def __init__(self, A, rate, G, Qm=2, rv=0, bg=0, device=0):
+        import numpy as np
+        h = _P()
+        _check(lib.ldpc_tb_create(ctypes.byref(h), A, rate, bg, G, Qm, rv, device), "ldpc_tb_create")
+        self._h = h
+        inf = TbInfo()
+        E = np.zeros(4096, dtype=np.int32)
+        _check(lib.ldpc_tb_get_info(h, ctypes.byref(inf), E.ctypes.data_as(_c_int_p), E.size), "ldpc_tb_get_info")
+        self.info = {f: getattr(inf, f) for f, _ in TbInfo._fields_}
+        for k, v in self.info.items():
+            setattr(self, k, v)
+        self.E = E[:self.C].copy()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib.ldpc_tb_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode_host(self, a_bits):
+        """payload bits [A] -> (transmitted bits [G], code blocks [C][n_vn])."""
+        import numpy as np
+        a = np.ascontiguousarray(a_bits, dtype=np.uint8)
+        assert a.size == self.A
+        f = np.empty(self.G, dtype=np.uint8)
+        cw = np.empty((self.C, self.n_vn), dtype=np.uint8)
+        _check(lib.ldpc_tb_encode_host(self._h, a.ctypes.data, f.ctypes.data, cw.ctypes.data), "ldpc_tb_encode_host")
+        return f, cw
+
+    def derate(self, llr_in_ptr, llr_cb_ptr, stream=0):
+        _check(lib.ldpc_tb_derate_async(self._h, stream, llr_in_ptr, llr_cb_ptr), "ldpc_tb_derate_async")
+
+    def check(self, hd_ptr, cb_ok_ptr, tb_bits_ptr=0, stream=0):
+        _check(lib.ldpc_tb_check_async(self._h, stream, hd_ptr, cb_ok_ptr, tb_bits_ptr or None), "ldpc_tb_check_async")

@@ -37,7 +37,8 @@ COMMON = [
 ] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
 SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
-           "encoder.cpp", "noms_api.cpp", "compat.cpp", "mixed_plan.cpp", "mixed.hip"]
+           "encoder.cpp", "noms_api.cpp", "compat.cpp", "mixed_plan.cpp", "mixed.hip",
+           "tb_host.cpp", "tb.hip"]
 DRIVER_SRC = os.path.join(HERE, "driver", "ldpc_ber.cpp")
 DRIVER = os.path.join(HERE, "bin", "ldpc_ber")
 
