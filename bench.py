@@ -11,6 +11,8 @@ Also measured, each in its own object of the same JSON line:
   * ``combined_stream``: BG1 Z=384 and BG2 Z=208 codewords at mixed rates in batches of
     1..8192, each batch a replayed HIP graph (LLR upload, plan upload, decode, hard-decision
     download), plain min-sum against learned weights with plain fallback;
+  * ``stream_overlap``: the same traffic as batches of 16 / 256 / 2048 codewords over 1, 2 and 4
+    streams round-robin (host planning and PCIe copies overlapped with decoding);
   * ``cpu_baseline``: the C oracle (same schedule) on this host's cores.
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--skip-mixed] [--skip-stream]
@@ -334,6 +336,80 @@ def bench_stream(torch, nr, args):
     return out
 
 
+def bench_pipeline(torch, nr, args):
+    """Copy/compute overlap over S = 1, 2, 4 streams: batches of c codewords (BG1 Z=384 / BG2
+    Z=208 at mixed rates, Eb/N0 1.5 dB) submitted round-robin, each one replayed graph (LLR
+    upload from pinned host memory, plan upload, persistent decode with early exit, hard-decision
+    download) on its stream's own buffers and mixed object; the host plans batch i + S while
+    the GPU works on batch i.  Every slot's decisions are checked against a decode of the same
+    batch with nothing else in flight."""
+    from nr_gpu_ldpc_amd import mixed as mx
+    T = 10
+    S_MAX = 4
+    rng = np.random.default_rng(7)
+    streams = [torch.cuda.Stream() for _ in range(S_MAX)]
+    contexts = {}
+    out = {"workload": "BG1 Z=384 / BG2 Z=208 (50/50) at rates U[1/5, 8/9], Eb/N0 1.5 dB, T=10 with "
+                       "early exit; per batch: plan on the host, then one replayed HIP graph (pinned "
+                       "LLR upload, plan upload, decode, hard-decision download); S streams round-robin",
+           "latency_kind": "device time per batch on its stream (start to end event), under S-way overlap"}
+    for c in (16, 256, 2048):
+        slots = []
+        for k in range(S_MAX):
+            kind = np.where(rng.random(c) < 0.5, nr.BG1, nr.BG2)
+            Zs = np.where(kind == nr.BG1, 384, 208)
+            r_nom = rng.uniform(0.2, 8 / 9, c)
+            mr = np.where(kind == nr.BG1, mx.rows_for_rate(nr.BG1, r_nom), mx.rows_for_rate(nr.BG2, r_nom))
+            b = mx.describe(kind, Zs, mr, ebn0=np.full(c, 1.5))
+            llr, _ = mx.make_channel(torch, b, seed=50000 + 10 * c + k, contexts=contexts)
+            h_llr = torch.empty(b["llr_floats"], dtype=torch.float32).pin_memory()
+            h_llr.copy_(llr.cpu())
+            h_hd = torch.zeros(b["hd_total"], dtype=torch.uint8).pin_memory()
+            d_llr = torch.empty_like(llr)
+            d_hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
+            m = nr.Mixed(c)
+            m.plan(b["desc"])
+            st = streams[k].cuda_stream
+            m.capture(st, d_llr.data_ptr(), T, d_hd.data_ptr(), 0, 0, flags=nr.EARLY_EXIT,
+                      llr_host_ptr=h_llr.data_ptr(), llr_floats=b["llr_floats"], hd_host_ptr=h_hd.data_ptr(),
+                      hd_bytes=b["hd_total"])
+            m.replay(st)
+            streams[k].synchronize()
+            slots.append({"m": m, "b": b, "h_llr": h_llr, "h_hd": h_hd, "d_llr": d_llr, "d_hd": d_hd,
+                          "want": h_hd.clone(), "info": int(b["info_bits"].sum())})
+        row = {}
+        for S in (1, 2, 4):
+            reps = max(8 * S, min(256, (1 << 17) // c))
+            for k in range(S):  # warm-up, one per stream
+                slots[k]["m"].plan(slots[k]["b"]["desc"])
+                slots[k]["m"].replay(streams[k].cuda_stream)
+            torch.cuda.synchronize()
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+            info = 0
+            t0 = time.perf_counter()
+            for i in range(reps):
+                sl, st = slots[i % S], streams[i % S]
+                sl["m"].plan(sl["b"]["desc"])
+                ev[i][0].record(st)
+                sl["m"].replay(st.cuda_stream)
+                ev[i][1].record(st)
+                info += sl["info"]
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            for k in range(S):
+                if not torch.equal(slots[k]["h_hd"], slots[k]["want"]):
+                    raise RuntimeError(f"pipelined decode differs from the lone decode (c={c}, S={S}, slot {k})")
+            lat = [a.elapsed_time(b_) for a, b_ in ev]
+            row[f"streams_{S}"] = {"batches": reps, "info_gbps": round(info / wall / 1e9, 3),
+                                   "batches_per_s": round(reps / wall, 1),
+                                   "latency_ms": {"p50": round(statistics.median(lat), 3),
+                                                  "p99": round(pct(lat, 0.99), 3)}}
+        out[f"batch_{c}"] = row
+        for sl in slots:
+            sl["m"].close()
+    return out
+
+
 def bench_operating_point(torch, nr, args):
     """Info throughput at the FER 1e-3 operating point of each decoder (BG1 Z=384, T <= 10 with
     early exit, batches of 4096): plain min-sum at its FER 1e-3 Eb/N0, the learned decoder (with
@@ -458,6 +534,8 @@ def main():
         out["mixed_batch"] = bench_mixed(torch, nr, args)
     if not args.skip_stream:
         out["combined_stream"] = bench_stream(torch, nr, args)
+    if not args.skip_stream:
+        out["stream_overlap"] = bench_pipeline(torch, nr, args)
     if not args.skip_stream:
         ex = bench_reference_example()
         if ex:

@@ -88,6 +88,9 @@ int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, i
  * single-precision posterior). */
 int ldpc_decode_half_async(ldpc_ctx* ctx, void* stream, const uint16_t* llr_half, int n_cw, int n_iter,
                            unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+/* int8 LLRs (as the reference's quantize(D, x, 8) makes them, src/host_helpers.cpp:46-58) ->
+ * float LLRs out[i] = in[i] * scale (scale = D), n values. */
+int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float scale, float* out);
 /* float LLRs -> binary16 bit patterns (round to nearest even), n values; the reference's
  * reduce_precision<half>, src/setup_exp.cu:33-38, on the device. */
 int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out);

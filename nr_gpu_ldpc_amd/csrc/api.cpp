@@ -392,6 +392,11 @@ int ldpc_decode_half_async(ldpc_ctx* x, void* stream, const uint16_t* llr, int n
     return decode_half(x, stream, llr, n_cw, n_iter, flags, hd, app_out, iters_out);
 }
 
+int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float scale, float* out) {
+    if ((!in || !out) && n) return LDPC_EINVAL;
+    return from_int8_launch(in, n, scale, out, static_cast<hipStream_t>(stream));
+}
+
 int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out) {
     if ((!llr || !out) && n) return LDPC_EINVAL;
     return to_half_launch(llr, n, out, static_cast<hipStream_t>(stream));

@@ -47,6 +47,8 @@ struct StreamPackArgs {
 
 int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s);
 int stream_pack_launch(const StreamPackArgs& a, hipStream_t s);
+// int8 LLRs -> float, scaled.
+int from_int8_launch(const int8_t* in, size_t n, float scale, float* out, hipStream_t s);
 // float -> IEEE binary16 bit patterns, round to nearest even.
 int to_half_launch(const float* in, size_t n, uint16_t* out, hipStream_t s);
 int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s);

@@ -251,6 +251,19 @@ __global__ void __launch_bounds__(256) k_to_half(const float* in, size_t n, uint
         out[i] = __builtin_bit_cast(uint16_t, (_Float16)in[i]);
 }
 
+__global__ void __launch_bounds__(256) k_from_int8(const int8_t* in, size_t n, float scale, float* out) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i] * scale;
+}
+
+int from_int8_launch(const int8_t* in, size_t n, float scale, float* out, hipStream_t s) {
+    if (n == 0) return LDPC_OK;
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_from_int8, dim3((unsigned)blocks), dim3(256), 0, s, in, n, scale, out);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
 int to_half_launch(const float* in, size_t n, uint16_t* out, hipStream_t s) {
     if (n == 0) return LDPC_OK;
     const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);

@@ -19,6 +19,27 @@
 namespace {
 
 ldpc_ctx* g_ctx = nullptr;  // the process-wide code, as the reference's __constant__ memory
+int g_n_cw = 0, g_n_iter = 0, g_n_vn = 0;  // batch geometry of that code (ldpc_gpu_decoder)
+
+// Device buffers of ldpc_gpu_decoder, sized for the process-wide geometry.
+struct CEntryBuffers {
+    int8_t* in = nullptr;
+    float* llr = nullptr;
+    uint8_t* hd = nullptr;
+    uint8_t* packed = nullptr;
+    hipStream_t stream = nullptr;
+    size_t n = 0;
+};
+CEntryBuffers g_cbuf;
+
+void free_centry_buffers() {
+    if (g_cbuf.in) (void)hipFree(g_cbuf.in);
+    if (g_cbuf.llr) (void)hipFree(g_cbuf.llr);
+    if (g_cbuf.hd) (void)hipFree(g_cbuf.hd);
+    if (g_cbuf.packed) (void)hipFree(g_cbuf.packed);
+    if (g_cbuf.stream) (void)hipStreamDestroy(g_cbuf.stream);
+    g_cbuf = CEntryBuffers{};
+}
 
 [[noreturn]] void die_ldpc(int st, const char* what) {
     fprintf(stderr, "nr_gpu_ldpc: %s failed: %s (%s)\n", what, ldpc_strerror(st), ldpc_last_error());
@@ -343,11 +364,15 @@ void generate_new_codewords(const params* p, host_mem<T>* h, const ldpc_params* 
 }
 
 void put_h_compact_and_params_in_constant_memory(const ldpc_params* ldpc, const params* /*p*/,
-                                                 const cuda_grid* /*g*/) {
+                                                 const cuda_grid* g) {
     if (g_ctx) {
         ldpc_ctx_destroy(g_ctx);
         g_ctx = nullptr;
     }
+    free_centry_buffers();
+    g_n_cw = g ? (int)(g->cnp_blocks.y * g->cnp_threads.y) : 0;
+    g_n_iter = ldpc->n_iterations;
+    g_n_vn = ldpc->N_before_puncture;
     int dev = 0;
     NRLDPC_CHECK_HIP(hipGetDevice(&dev));
     NRLDPC_CHECK(ldpc_ctx_create(&g_ctx, ldpc->bg_index, ldpc->Z, ldpc->H_rows, dev));
@@ -368,6 +393,30 @@ void perform_gpu_ldpc_decoding(dev_mem<T>* d, host_mem<T>* h, const cuda_grid* g
     if (per_cw != packed) NRLDPC_CHECK(ldpc_hd_to_stream_async(g_ctx, st, per_cw, n_cw, packed));
     NRLDPC_CHECK_HIP(hipMemcpyAsync(h->hd_packed_cuda[stream_index], packed, h->mem_size_hd_cuda_packed,
                                     hipMemcpyDeviceToHost, st));
+}
+
+extern "C" void ldpc_gpu_decoder(char* inputs, char* outputs) {
+    if (!g_ctx || g_n_cw < 1) config_error("ldpc_gpu_decoder before put_h_compact_and_params_in_constant_memory");
+    const size_t n = (size_t)g_n_cw * g_n_vn;
+    const size_t hb = (size_t)(g_n_vn + 7) / 8, packed = (n + 7) / 8;
+    if (g_cbuf.n != n) {
+        free_centry_buffers();
+        NRLDPC_CHECK_HIP(hipStreamCreate(&g_cbuf.stream));
+        NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.in), n));
+        NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.llr), n * sizeof(float)));
+        NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.hd), (size_t)g_n_cw * hb));
+        NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.packed), packed));
+        g_cbuf.n = n;
+    }
+    const hipStream_t s = g_cbuf.stream;
+    NRLDPC_CHECK_HIP(hipMemcpyAsync(g_cbuf.in, inputs, n, hipMemcpyHostToDevice, s));
+    NRLDPC_CHECK(ldpc_llr_from_int8_async(s, g_cbuf.in, n, 0.5f, g_cbuf.llr));
+    const bool repack = g_n_vn % 8 != 0;
+    uint8_t* hd = repack ? g_cbuf.hd : g_cbuf.packed;
+    NRLDPC_CHECK(ldpc_decode_async(g_ctx, s, g_cbuf.llr, g_n_cw, g_n_iter, 0, hd, nullptr, nullptr));
+    if (repack) NRLDPC_CHECK(ldpc_hd_to_stream_async(g_ctx, s, hd, g_n_cw, g_cbuf.packed));
+    NRLDPC_CHECK_HIP(hipMemcpyAsync(outputs, g_cbuf.packed, packed, hipMemcpyDeviceToHost, s));
+    NRLDPC_CHECK_HIP(hipStreamSynchronize(s));
 }
 
 template host_mem<float> setup_host<float>(const params*, unsigned int, unsigned int);

@@ -63,6 +63,6 @@ def test_cpp_host_api_exported(nr):
             "void generate_new_codewords<float>(params const*, host_mem<float>*, ldpc_params const*, int, float)",
             "put_h_compact_and_params_in_constant_memory(ldpc_params const*, params const*, cuda_grid const*)",
             "void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, cuda_grid const*, int, int)",
-            "nr_gpu_ldpc_context()"]
+            "nr_gpu_ldpc_context()", "ldpc_gpu_decoder"]
     missing = [w for w in want if w not in out]
     assert not missing, missing

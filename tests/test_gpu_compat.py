@@ -71,3 +71,18 @@ def test_stream_packing_for_odd_lifting_size(nr, torch_gpu):
     torch.cuda.synchronize()
     assert code.n_vn % 8 != 0
     assert np.array_equal(d_out.cpu().numpy(), np.packbits(bits.reshape(-1), bitorder="little"))
+
+
+def test_c_entry_point_matches_c_abi(tmp_path):
+    """ldpc_gpu_decoder(char*, char*) on int8 LLRs equals the C ABI decode of 0.5 * q."""
+    print("case C entry point BG1 Z=384", flush=True)
+    exe = str(tmp_path / "c_entry")
+    lib = os.path.join(ROOT, "nr_gpu_ldpc_amd", "lib")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", f"-I{ROOT}/include",
+                    f"{ROOT}/tests/compat/c_entry.cpp", f"-L{lib}", "-lnrldpc", f"-Wl,-rpath,{lib}", "-o", exe],
+                   check=True, capture_output=True, timeout=120)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["status"] == 0 and out["differing_bytes"] == 0 and out["bytes"] == (8 * 26112 + 7) // 8
+    assert out["ones"] > 0

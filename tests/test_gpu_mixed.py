@@ -104,6 +104,40 @@ def test_mixed_graph_replay_follows_the_plan(nr, torch_gpu):
         assert np.array_equal(d_it.cpu().numpy()[:len(b["kind"])], want[2]), f"replay {rep}"
 
 
+def test_concurrent_streams_match_lone_decodes(nr, torch_gpu):
+    """Three mixed objects, each with its graph captured on its own stream, replayed round-robin
+    with several batches in flight at once: every stream's decisions and iteration counts equal
+    a decode of its batch with nothing else running."""
+    torch = torch_gpu
+    print("case mixed concurrent streams", flush=True)
+    slots = []
+    for k in range(3):
+        b = mx.draw_batch(60 + 40 * k, seed=70 + k, sizes=[16, 52, 208, 384], ebn0=(0.5, 3.0))
+        llr, _ = mx.make_channel(torch, b, seed=80 + k)
+        want = _mixed(nr, torch, b, llr, 10, nr.EARLY_EXIT)
+        h_llr = torch.zeros(b["llr_floats"], dtype=torch.float32).pin_memory()
+        h_llr.copy_(llr.cpu())
+        h_hd = torch.zeros(b["hd_total"], dtype=torch.uint8).pin_memory()
+        d_llr = torch.zeros(b["llr_floats"], dtype=torch.float32, device="cuda")
+        d_hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
+        d_it = torch.zeros(len(b["kind"]), dtype=torch.int32, device="cuda")
+        s = torch.cuda.Stream()
+        m = nr.Mixed(len(b["kind"]))
+        m.plan(b["desc"])
+        m.capture(s.cuda_stream, d_llr.data_ptr(), 10, d_hd.data_ptr(), 0, d_it.data_ptr(), flags=nr.EARLY_EXIT,
+                  llr_host_ptr=h_llr.data_ptr(), llr_floats=b["llr_floats"], hd_host_ptr=h_hd.data_ptr(),
+                  hd_bytes=b["hd_total"])
+        slots.append((b, m, s, h_hd, d_it, want, d_llr))
+    for rep in range(12):
+        b, m, s = slots[rep % 3][:3]
+        m.plan(b["desc"])
+        m.replay(s.cuda_stream)
+    torch.cuda.synchronize()
+    for k, (b, m, s, h_hd, d_it, want, _) in enumerate(slots):
+        assert np.array_equal(h_hd.numpy(), want[0][:b["hd_total"]]), f"stream {k}"
+        assert np.array_equal(d_it.cpu().numpy(), want[2]), f"stream {k}"
+
+
 def test_autotune_picks_a_timed_candidate(nr, torch_gpu):
     torch = torch_gpu
     print("case mixed autotune BG2 Z=52", flush=True)
