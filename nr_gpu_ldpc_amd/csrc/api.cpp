@@ -224,7 +224,8 @@ int ldpc_decode_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int
                       uint8_t* hd_packed, float* app_out, int* iters_out) {
     if (!x || !llr || !hd_packed || n_cw < 0 || n_iter < 0) return LDPC_EINVAL;
     if (n_cw == 0) return LDPC_OK;
-    return decode_with(x, stream, llr, n_cw, n_iter, flags, hd_packed, app_out, iters_out, nullptr, nullptr);
+    return decode_with(x, stream, llr, n_cw, n_iter, flags, hd_packed, app_out, iters_out, nullptr, nullptr,
+                       nullptr);
 }
 
 int ldpc_syndrome_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, int* weight) {
@@ -241,6 +242,7 @@ int ldpc_syndrome_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, 
     a.edge_col = x->d_edge_col;
     a.edge_vm = x->d_edge_vm;
     a.weight = weight;
+    a.n_edges = x->code.nnz();
     return syndrome_launch(a, n_cw, static_cast<hipStream_t>(stream));
 }
 

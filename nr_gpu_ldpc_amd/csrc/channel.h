@@ -28,6 +28,7 @@ struct SyndromeArgs {
     const int* edge_col;   // per edge, row-major
     const int* edge_vm;
     int* weight;
+    int n_edges;           // row_start[rows]
 };
 
 struct ErrorArgs {
@@ -51,6 +52,8 @@ int stream_pack_launch(const StreamPackArgs& a, hipStream_t s);
 int from_int8_launch(const int8_t* in, size_t n, float scale, float* out, hipStream_t s);
 // float -> IEEE binary16 bit patterns, round to nearest even.
 int to_half_launch(const float* in, size_t n, uint16_t* out, hipStream_t s);
+// Ascending indices of the codewords whose weight is nonzero, and their count (device).
+int compact_failures_launch(const int* weight, int n_cw, int* list, int* count, hipStream_t s);
 int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s);
 int count_errors_launch(const ErrorArgs& a, int n_cw, hipStream_t s);
 

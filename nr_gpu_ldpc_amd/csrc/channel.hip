@@ -172,24 +172,39 @@ __global__ void __launch_bounds__(256) k_channel(ChannelArgs a) {
     }
 }
 
-// Syndrome weight per codeword from packed hard decisions.
+// Syndrome weight per codeword from packed hard decisions.  The codeword's packed decisions
+// and the code's edge list are staged in LDS (coalesced loads), so every parity check gathers
+// its bits from LDS.  Edge e is kept as (block column * Z) | (V mod Z) << 16.
 __global__ void __launch_bounds__(256) k_syndrome(SyndromeArgs a) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char sm[];
     const int cw = blockIdx.x;
+    const int Z = a.Z, rows = a.rows;
+    int* rs = reinterpret_cast<int*>(sm);                 // [rows + 1]
+    uint32_t* et = reinterpret_cast<uint32_t*>(rs + rows + 1);
+    const int n_e = a.row_start[rows];
+    uint8_t* hb = reinterpret_cast<uint8_t*>(et + n_e);    // [hd_bytes]
+    for (int i = threadIdx.x; i <= rows; i += blockDim.x) rs[i] = a.row_start[i];
+    for (int e = threadIdx.x; e < n_e; e += blockDim.x)
+        et[e] = (uint32_t)(a.edge_col[e] * Z) | ((uint32_t)a.edge_vm[e] << 16);
     const uint8_t* hd = a.hd + (size_t)cw * a.hd_bytes;
-    const int Z = a.Z;
+    for (int b = threadIdx.x; b < a.hd_bytes; b += blockDim.x) hb[b] = hd[b];
+    __syncthreads();
     int w = 0;
-    for (int chk = threadIdx.x; chk < a.rows * Z; chk += blockDim.x) {
+    for (int chk = threadIdx.x; chk < rows * Z; chk += blockDim.x) {
         const int i = chk / Z, r = chk - i * Z;
         uint32_t p = 0;
-        for (int e = a.row_start[i]; e < a.row_start[i + 1]; e++) {
-            int c = r + a.edge_vm[e];
+        const int e1 = rs[i + 1];
+        for (int e = rs[i]; e < e1; e++) {
+            const uint32_t t = et[e];
+            int c = r + (int)(t >> 16);
             if (c >= Z) c -= Z;
-            const int n = a.edge_col[e] * Z + c;
-            p ^= (hd[n >> 3] >> (n & 7)) & 1u;
+            const int n = (int)(t & 0xFFFFu) + c;
+            p ^= (uint32_t)(hb[n >> 3] >> (n & 7));
         }
-        w += (int)p;
+        w += (int)(p & 1u);
     }
-    __shared__ int red[256];
+    __syncthreads();
+    int* red = reinterpret_cast<int*>(sm);
     red[threadIdx.x] = w;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -197,6 +212,36 @@ __global__ void __launch_bounds__(256) k_syndrome(SyndromeArgs a) {
         __syncthreads();
     }
     if (threadIdx.x == 0) a.weight[cw] = red[0];
+}
+
+// Indices of the codewords with a nonzero syndrome weight, in ascending order, and their
+// count (one workgroup; block-wide prefix sums over chunks of 1024).
+__global__ void __launch_bounds__(1024) k_compact_failures(const int* __restrict__ weight, int n_cw,
+                                                            int* __restrict__ list, int* __restrict__ count) {
+    __shared__ int warp_sums[16];
+    __shared__ int base_s;
+    if (threadIdx.x == 0) base_s = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < n_cw; c0 += 1024) {
+        const int i = c0 + (int)threadIdx.x;
+        const bool f = i < n_cw && weight[i] != 0;
+        const uint64_t m = __ballot(f);
+        const int below = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) warp_sums[wv] = __popcll(m);
+        __syncthreads();
+        int off = base_s;
+        for (int k = 0; k < wv; k++) off += warp_sums[k];
+        if (f) list[off + below] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int t = 0;
+            for (int k = 0; k < 16; k++) t += warp_sums[k];
+            base_s += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = base_s;
 }
 
 // Bit and frame errors over the first K columns.
@@ -280,7 +325,16 @@ int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s) {
 }
 
 int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s) {
-    hipLaunchKernelGGL(k_syndrome, dim3(n_cw), dim3(256), 0, s, a);
+    const size_t lds = std::max<size_t>(256 * sizeof(int),
+                                        (size_t)(a.rows + 1) * sizeof(int) + (size_t)a.n_edges * sizeof(uint32_t) +
+                                            (size_t)a.hd_bytes);
+    hipLaunchKernelGGL(k_syndrome, dim3(n_cw), dim3(256), lds, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
+int compact_failures_launch(const int* weight, int n_cw, int* list, int* count, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_failures, dim3(1), dim3(1024), 0, s, weight, n_cw, list, count);
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;
 }

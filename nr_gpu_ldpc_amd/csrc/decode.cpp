@@ -8,7 +8,8 @@
 namespace nrldpc {
 
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
-                uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* select) {
+                uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,
+                const int* list_count) {
     if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
     NRLDPC_HIP(hipSetDevice(x->device));
     const CodeDesc& c = x->code;
@@ -29,7 +30,8 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.lds_cw_bytes = (int)decoder_lds_bytes_per_cw(c.kind, c.Z);
     a.nnz = nnz_full(c.kind);
-    a.select = select;
+    a.list = list;
+    a.list_count = list_count;
     return decoder_launch(c.kind, weights != nullptr, a, static_cast<hipStream_t>(stream));
 }
 

@@ -36,7 +36,8 @@ struct DecodeArgs {
     float* rec_m1;
     float* rec_m2;
     uint32_t* rec_w;       // Q sign bits (edge k at bit d-1-k) | arg-min << 20 | 2nd arg-min << 25
-    const int* select;     // [n_cw] or null: decode only codewords with select[cw] != 0
+    const int* list;       // or null: decode only codewords list[0 .. *list_count - 1]
+    const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
 };
 

@@ -86,6 +86,19 @@ __device__ __forceinline__ bool block_any(bool v, uint32_t slot) {
     return any;
 }
 
+// Two workgroup-wide "any lane true" votes at once (bit 0: a, bit 1: b).
+__device__ __forceinline__ uint32_t block_any2(bool a, bool b, uint32_t slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) lds_stu(slot, 0u);
+    __syncthreads();
+    const uint32_t v = (a ? 1u : 0u) | (b ? 2u : 0u);
+    if (v) __hip_atomic_fetch_or((lds_u32*)(uintptr_t)slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    const uint32_t m = lds_ldu(slot);
+    __syncthreads();
+    return m;
+}
+
 // binary16 storage of the HALF_PRC schedule: round to nearest even and back.
 __device__ __forceinline__ float round_half(float x) { return (float)(_Float16)x; }
 __device__ __forceinline__ float half_bits_to_float(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
@@ -188,16 +201,29 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     if constexpr (MIXED) {
         if (cwl >= a.cpb) { dead = true; cwl = a.cpb - 1; }
     }
-    const int cw = blk * a.cpb + cwl;
+    int cw = blk * a.cpb + cwl;
     bool live;
     int cwc;
     if constexpr (MIXED) {
         live = !dead && cwl < a.n_cw && sel;
         cwc = cwl < a.n_cw ? cwl : a.n_cw - 1;  // index into mcw
+    } else if (a.list != nullptr) {
+        // decode only the listed codewords: slot blk * cpb + cwl takes list[slot]
+        const int n_list = *a.list_count;
+        if (blk * a.cpb >= n_list) return false;  // uniform over the workgroup
+        live = cw < n_list;
+        cw = live ? a.list[cw] : 0;
+        cwc = live ? cw : a.n_cw - 1;
     } else {
-        live = cw < a.n_cw && (a.select == nullptr || a.select[cw] != 0);
+        live = cw < a.n_cw;
         cwc = live ? cw : a.n_cw - 1;
     }
+    // One codeword per workgroup (and no mixed tile): every per-codeword flag is uniform over
+    // the workgroup, so no workgroup vote is needed.
+    const bool uniform_cw = !MIXED && a.cpb == 1;
+    // All lanes of this wavefront decode the same codeword (dead lanes count as the codeword
+    // they shadow).
+    const bool wave_one_cw = __ballot(cwl != __builtin_amdgcn_readfirstlane(cwl)) == 0u;
     int m_lane = a.m_rows;  // parity rows of this lane's codeword
     if constexpr (MIXED) m_lane = mcw[cwc].m_rows;
     // MIXED: a wavefront whose lanes are all beyond cpb*Z only takes part in the barriers.
@@ -213,7 +239,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     const uint32_t wgFlag = lds_base() + (uint32_t)a.cpb * a.lds_cw_bytes;  // workgroup flag slot
     CEdge* et = (CEdge*)(a.etab);
     const int m_rows = a.m_rows;
-    if (a.select != nullptr && !block_any(live, wgFlag)) return false;  // nothing to decode here
 
     constexpr int NW = S::NW;
     float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
@@ -508,29 +533,38 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         if (check && r == 0 && !idle) lds_stu(flagAddr, 0u);
         __syncthreads();
         if (check) {
-            // syndrome of the current hard decision; any unsatisfied check keeps decoding
-            uint32_t bad = 0u;
+            // Syndrome of the current hard decision; any unsatisfied check keeps decoding.  The
+            // parity of a check row is taken over wavefront ballots of "APP > 0" (hard decision
+            // 0): one compare per edge, the XOR of the 64-lane masks on the scalar unit.
+            // A wavefront whose lanes all belong to one codeword stops at its first row with an
+            // unsatisfied check (a uniform branch): before convergence that is almost always
+            // one of the first rows.
+            uint64_t badm = 0u;
             if (!idle) static_for<0, ROWS>([&](auto IC) {
                 constexpr int i = IC;
                 constexpr int D = S::deg(i);
                 constexpr int E0 = S::T.row_start[i];
                 int mr = m_rows;
                 asm volatile("" : "+s"(mr));
-                if (i >= mr) return;
+                if (i >= mr || (wave_one_cw && badm != 0u)) return;
                 asm volatile("" : "+s"(et));
-                uint32_t p = 0u;
+                uint64_t zeros = 0u;  // lanes with an odd number of hard-decision zeros
                 static_for<0, D>([&](auto KC) {
                     constexpr int e = E0 + KC;
                     constexpr int j = S::T.ecol[e];
                     float x;
                     if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
                     else x = app1[S::T.d1_slot[j]];
-                    p ^= x > 0.f ? 0u : 1u;
+                    zeros ^= __ballot(x > 0.f);
                 });
-                if constexpr (MIXED) bad |= (i < m_lane) ? p : 0u;
-                else bad |= p;
+                // ones = D - zeros: odd parity of the ones where that of the zeros differs from D's
+                // (complemented masks are cut to the active lanes of a partial last wavefront)
+                uint64_t odd = (D & 1) ? (~zeros & __ballot(true)) : zeros;
+                if constexpr (MIXED) odd &= __ballot(i < m_lane);
+                badm |= odd;
             });
-            if (bad) lds_stu(flagAddr, 1u);
+            const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+            if ((badm >> lane) & 1u) lds_stu(flagAddr, 1u);
             __syncthreads();
             const bool conv = lds_ldu(flagAddr) == 0u;
             if (fin) {
@@ -538,17 +572,27 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 break;
             }
             const bool fresh = live && conv && !emitted;
-            if (block_any(fresh, wgFlag)) {
+            const bool rest = live && !emitted && !fresh;
+            bool any_fresh, any_rest;
+            if (uniform_cw) {  // one codeword per workgroup: both are workgroup-uniform already
+                any_fresh = __builtin_amdgcn_readfirstlane((int)fresh) != 0;
+                any_rest = __builtin_amdgcn_readfirstlane((int)rest) != 0;
+            } else {
+                const uint32_t m = block_any2(fresh, rest, wgFlag);
+                any_fresh = (m & 1u) != 0u;
+                any_rest = (m & 2u) != 0u;
+            }
+            if (any_fresh) {
                 // staging reuses the read copy of the converged codewords only
                 emit(fresh, t + 1);
                 if (fresh) emitted = true;
             }
-            if (!block_any(live && !emitted, wgFlag)) break;
+            if (!any_rest) break;
         }
     }
     bool out = live && !emitted;
     if constexpr (MIXED) out = out && (!emit_only_ok || final_ok);
-    if (block_any(out, wgFlag)) emit(out, t < a.n_iter ? t + 1 : a.n_iter);
+    if (uniform_cw ? __builtin_amdgcn_readfirstlane((int)out) != 0 : block_any(out, wgFlag)) emit(out, t < a.n_iter ? t + 1 : a.n_iter);
     return emitted || final_ok;
 }
 

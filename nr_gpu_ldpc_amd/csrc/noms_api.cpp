@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "../../include/ldpc_api.h"
+#include "channel.h"
 #include "context.h"
 #include "decoder.h"
 #include "learned.h"
@@ -76,7 +77,7 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     }
     if (n_cw == 0) return LDPC_OK;
     int st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
-                         x->d_weights, nullptr);
+                         x->d_weights, nullptr, nullptr);
     if (st || !(flags & LDPC_FLAG_FALLBACK_PLAIN)) return st;
     // Fallback: plain min-sum re-decodes the codewords whose learned hard decision fails the
     // syndrome check, into scratch; its outputs replace the learned ones only where the plain
@@ -86,7 +87,9 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t o_w2 = up((size_t)n_cw * sizeof(int));
     const size_t o_it = o_w2 + up((size_t)n_cw * sizeof(int));
-    const size_t o_hd = o_it + up((size_t)n_cw * sizeof(int));
+    const size_t o_list = o_it + up((size_t)n_cw * sizeof(int));
+    const size_t o_cnt = o_list + up((size_t)n_cw * sizeof(int));
+    const size_t o_hd = o_cnt + 256;
     const size_t o_app = o_hd + up((size_t)n_cw * c.hd_bytes());
     const size_t total = o_app + (app_out ? (size_t)n_cw * c.n_vn() * sizeof(float) : 0);
     void* scratch = nullptr;
@@ -95,15 +98,20 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     int* w1 = reinterpret_cast<int*>(sp);
     int* w2 = reinterpret_cast<int*>(sp + o_w2);
     int* it2 = reinterpret_cast<int*>(sp + o_it);
+    int* list = reinterpret_cast<int*>(sp + o_list);
+    int* cnt = reinterpret_cast<int*>(sp + o_cnt);
     uint8_t* hd2 = reinterpret_cast<uint8_t*>(sp + o_hd);
     float* app2 = app_out ? reinterpret_cast<float*>(sp + o_app) : nullptr;
+    const hipStream_t hs = static_cast<hipStream_t>(stream);
     if ((st = ldpc_syndrome_async(x, stream, hd, n_cw, w1))) return st;
+    // the failing codewords, compacted, so that the re-decode's workgroups are all busy ones
+    if ((st = compact_failures_launch(w1, n_cw, list, cnt, hs))) return st;
     if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd2, app2, it2, nullptr,
-                          w1)))
+                          list, cnt)))
         return st;
     if ((st = ldpc_syndrome_async(x, stream, hd2, n_cw, w2))) return st;
     MergeArgs m{w1, w2, hd2, app2, it2, hd, app_out, iters_out, n_cw, c.hd_bytes(), c.n_vn()};
-    return fallback_merge_launch(m, static_cast<hipStream_t>(stream));
+    return fallback_merge_launch(m, hs);
 }
 
 size_t ldpc_noms_workspace_bytes(const ldpc_ctx* x, int n_cw, int n_iter) {

@@ -78,12 +78,22 @@ def test_codewords_per_block(nr, oracle, torch_gpu, cpb):
     assert np.array_equal(hd, ohd) and np.array_equal(app, oapp)
 
 
-def test_early_exit_matches_oracle(nr, oracle, torch_gpu):
-    print("case early exit BG1 Z=64", flush=True)
-    code = nr.Code(1, 64)
-    llr, _ = _noisy_llr(nr, code, 48, 2.0, 1 / 3, seed=5)
+# (kind, Z, codewords per workgroup or 0 for the default, codewords, Eb/N0, rate): the Z=52 and
+# Z=24 cases put lanes of two codewords into one wavefront, whose syndrome check must not stop
+# at the first failing row of either.
+EARLY_CASES = [(1, 64, 0, 48, 2.0, 1 / 3), (2, 52, 3, 40, 1.0, 0.2), (2, 24, 0, 60, 1.5, 0.2),
+               (1, 384, 1, 8, 1.6, 1 / 3)]
+
+
+@pytest.mark.parametrize("kind,Z,cpb,n_cw,ebn0,rate", EARLY_CASES)
+def test_early_exit_matches_oracle(nr, oracle, torch_gpu, kind, Z, cpb, n_cw, ebn0, rate):
+    print(f"case early exit kind={kind} Z={Z} cpb={cpb}", flush=True)
+    code = nr.Code(kind, Z)
+    if cpb:
+        code.set_launch(cpb)
+    llr, _ = _noisy_llr(nr, code, n_cw, ebn0, rate, seed=5 + Z)
     hd, app, iters = _gpu_decode(torch_gpu, code, llr, 20, flags=nr.EARLY_EXIT)
-    ohd, oapp, oit = oracle.decode(oracle.code(OR_BG1, 64), llr, 20, early_exit=True)
+    ohd, oapp, oit = oracle.decode(oracle.code(KIND[kind], Z), llr, 20, early_exit=True)
     assert np.array_equal(iters, oit)
     assert np.array_equal(hd, ohd) and np.array_equal(app, oapp)
     assert iters.min() < 20
