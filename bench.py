@@ -41,6 +41,9 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 PEAK_VALU_GOPS = 256 * 64 * 2.4
 VALU_OPS_PER_EDGE = 8  # convention: sub, min, med3, sign, select, sign-apply, dt-sub, APP-add
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_decode_bg1_z384_t10_b4096_traffic.json")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "k_decode_bg1_z384_t10_b4096_sq.json")
+# A CU issues at most one 64-lane VALU instruction per clock (4 SIMDs x 16 lanes).
+PEAK_VALU_ISSUE = 256 * 2.4e9
 TUNING_FILE = os.path.join(ROOT, "nr_gpu_ldpc_amd", "tuning", "mixed_gfx950.txt")
 
 
@@ -155,6 +158,15 @@ def bench_headline(torch, nr, args):
     valu_gops = valu_ops / (ms * 1e6)
     hbm_gbs = alg_bytes / (ms * 1e6)
     traffic, tsrc = traffic_per_launch()
+    try:
+        with open(ISSUE_FILE) as f:
+            issue = json.load(f)
+        n_valu = issue["valu_insts_per_launch"]
+        issued = {"valu_insts_per_launch": n_valu, "lane_ops_per_edge_iteration": round(n_valu * 64 / (BATCH * edges * N_ITER), 2),
+                  "rate_per_s": round(n_valu / (ms * 1e-3), 1), "peak_per_s": PEAK_VALU_ISSUE,
+                  "frac": round(n_valu / (ms * 1e-3) / PEAK_VALU_ISSUE, 4), "source": issue["source"]}
+    except (OSError, ValueError, KeyError):
+        issued = None
     # the reference's HALF_PRC schedule on the same batch (binary16 LLRs, 2 bytes per LLR)
     llr_h = torch.empty((BATCH, code.n_vn), dtype=torch.int16, device=dev)
     nr.llr_to_half(llr.data_ptr(), llr.numel(), llr_h.data_ptr(), stream=s)
@@ -187,7 +199,8 @@ def bench_headline(torch, nr, args):
             "traffic_source": (tsrc or {}).get("source"),
             "valu": {"note": "the binding limit: min-sum is compare/select/add work on the VALU",
                      "achieved": round(valu_gops, 1), "peak": round(PEAK_VALU_GOPS, 1), "unit": "Gop/s",
-                     "frac": round(valu_gops / PEAK_VALU_GOPS, 4), "ops_per_edge": VALU_OPS_PER_EDGE},
+                     "frac": round(valu_gops / PEAK_VALU_GOPS, 4), "ops_per_edge": VALU_OPS_PER_EDGE,
+                     "issued": issued},
         },
     }
     return res, llr
@@ -336,6 +349,29 @@ def bench_stream(torch, nr, args):
     return out
 
 
+def hip_streams(torch, n):
+    """n fresh HIP streams created back to back (the runtime hands out its hardware queues
+    round-robin, so consecutive streams do not share one), wrapped for torch, and a function
+    that destroys them."""
+    import ctypes
+    import importlib.util
+    spec = importlib.util.find_spec("torch")
+    hip = ctypes.CDLL(os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so"))
+    handles = []
+    for _ in range(n):
+        h = ctypes.c_void_p()
+        if hip.hipStreamCreate(ctypes.byref(h)) != 0:
+            raise RuntimeError("hipStreamCreate failed")
+        handles.append(h)
+
+    def destroy():
+        torch.cuda.synchronize()
+        for h in handles:
+            hip.hipStreamDestroy(h)
+
+    return [torch.cuda.ExternalStream(h.value) for h in handles], destroy
+
+
 def bench_pipeline(torch, nr, args):
     """Copy/compute overlap over S = 1, 2, 4 streams: batches of c codewords (BG1 Z=384 / BG2
     Z=208 at mixed rates, Eb/N0 1.5 dB) submitted round-robin, each one replayed graph (LLR
@@ -347,7 +383,7 @@ def bench_pipeline(torch, nr, args):
     T = 10
     S_MAX = 4
     rng = np.random.default_rng(7)
-    streams = [torch.cuda.Stream() for _ in range(S_MAX)]
+    streams, destroy = hip_streams(torch, S_MAX)
     contexts = {}
     out = {"workload": "BG1 Z=384 / BG2 Z=208 (50/50) at rates U[1/5, 8/9], Eb/N0 1.5 dB, T=10 with "
                        "early exit; per batch: plan on the host, then one replayed HIP graph (pinned "
@@ -407,6 +443,7 @@ def bench_pipeline(torch, nr, args):
         out[f"batch_{c}"] = row
         for sl in slots:
             sl["m"].close()
+    destroy()
     return out
 
 

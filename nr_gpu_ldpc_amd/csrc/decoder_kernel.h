@@ -106,6 +106,7 @@ __device__ __forceinline__ float half_bits_to_float(uint16_t u) { return (float)
 // Per-edge table entries are read through the constant address space so that the compiler
 // issues scalar (s_load) reads of these wave-uniform values.
 using CEdge = const EdgeParam __attribute__((address_space(4)));
+using CW2 = const float2 __attribute__((address_space(4)));
 
 template <int KIND>
 constexpr bool nd_columns_leading() {
@@ -283,7 +284,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     bool emitted = false;
     bool final_ok = false;
     const bool early = a.early_exit != 0;
-    const float2* __restrict__ wt = a.weights;  // learned: [(n_iter+1)][nnz] (alpha, beta), row 0 zero
+    // learned: [(n_iter+1)][nnz] (alpha, beta), row 0 zero; read-only during the launch, so read
+    // through the constant address space (scalar loads into SGPRs, as for the edge table)
+    CW2* wt = (CW2*)(a.weights);
 
     // Writes hard decisions (and APP) of every lane's codeword for which `mine` holds.
     auto emit = [&](bool mine, int iters) {
@@ -350,8 +353,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
         // out of the iteration loop into long-lived VGPRs.
         asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW), "+v"(rRdHi));
-        const float2* __restrict__ w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
-        const float2* __restrict__ w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
+        CW2* w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
+        CW2* w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
         static_for<0, ROWS>([&](auto IC) {
             constexpr int i = IC;
             constexpr int D = S::deg(i);
@@ -389,8 +392,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     const float mo = (idxo == (uint32_t)k) ? a2 : a1;
                     float rv;
                     if constexpr (LEARNED) {
-                        const float2 w = w_old[e];
-                        const float mag = w.x * fmaxf(mo - w.y, 0.0f);
+                        const float wx = w_old[e].x, wy = w_old[e].y;
+                        const float mag = wx * fmaxf(mo - wy, 0.0f);
                         rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
                     } else {
                         rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
@@ -435,8 +438,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     idxn = eq ? (uint32_t)k : idxn;
                     float ms;
                     if constexpr (LEARNED) {
-                        const float2 w = w_new[e];
-                        const float mag = w.x * fmaxf((eq ? n2 : n1) - w.y, 0.0f);
+                        const float wx = w_new[e].x, wy = w_new[e].y;
+                        const float mag = wx * fmaxf((eq ? n2 : n1) - wy, 0.0f);
                         ms = as_f(as_u(mag) ^ sbit);
                     } else {
                         ms = eq ? n2s : n1s;
