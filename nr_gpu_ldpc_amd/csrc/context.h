@@ -44,7 +44,7 @@ void build_edge_params(const CodeDesc& c, std::vector<EdgeParam>* out);
 // Shared decode path; weights != null selects the learned decoder with [(n_iter+1)][nnz] weights.
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,
-                const int* list_count);
+                const int* list_count, int* fail_out = nullptr);
 int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,
                 float* app_out, int* iters_out);
 

@@ -9,7 +9,7 @@ namespace nrldpc {
 
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,
-                const int* list_count) {
+                const int* list_count, int* fail_out) {
     if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
     NRLDPC_HIP(hipSetDevice(x->device));
     const CodeDesc& c = x->code;
@@ -32,6 +32,7 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     a.nnz = nnz_full(c.kind);
     a.list = list;
     a.list_count = list_count;
+    a.fail_out = fail_out;
     return decoder_launch(c.kind, weights != nullptr, a, static_cast<hipStream_t>(stream));
 }
 

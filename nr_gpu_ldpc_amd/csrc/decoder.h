@@ -36,6 +36,8 @@ struct DecodeArgs {
     float* rec_m1;
     float* rec_m2;
     uint32_t* rec_w;       // Q sign bits (edge k at bit d-1-k) | arg-min << 20 | 2nd arg-min << 25
+    int* fail_out;         // [n_cw] or null: 0 where the output hard decision satisfies every
+                           // parity check, else 1 (checked after the last iteration)
     const int* list;       // or null: decode only codewords list[0 .. *list_count - 1]
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)

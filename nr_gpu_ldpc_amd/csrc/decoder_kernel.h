@@ -514,6 +514,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         const bool last = t + 1 == a.n_iter;
         bool fin = false;  // MIXED: syndrome of the final hard decision
         if constexpr (MIXED) fin = final_check && last;
+        else fin = a.fail_out != nullptr && last;
         if (last && !fin) break;
         // APP_new becomes the read copy of the next iteration.
         if (!idle) {
@@ -596,6 +597,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     bool out = live && !emitted;
     if constexpr (MIXED) out = out && (!emit_only_ok || final_ok);
     if (uniform_cw ? __builtin_amdgcn_readfirstlane((int)out) != 0 : block_any(out, wgFlag)) emit(out, t < a.n_iter ? t + 1 : a.n_iter);
+    if constexpr (!MIXED) {
+        if (a.fail_out != nullptr && live && r == 0) a.fail_out[cw] = (emitted || final_ok) ? 0 : 1;
+    }
     return emitted || final_ok;
 }
 

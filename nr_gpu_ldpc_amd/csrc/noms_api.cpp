@@ -76,9 +76,10 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
         return LDPC_EINVAL;
     }
     if (n_cw == 0) return LDPC_OK;
-    int st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
-                         x->d_weights, nullptr, nullptr);
-    if (st || !(flags & LDPC_FLAG_FALLBACK_PLAIN)) return st;
+    const bool fb = (flags & LDPC_FLAG_FALLBACK_PLAIN) != 0;
+    if (!fb)
+        return decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                           x->d_weights, nullptr, nullptr);
     // Fallback: plain min-sum re-decodes the codewords whose learned hard decision fails the
     // syndrome check, into scratch; its outputs replace the learned ones only where the plain
     // hard decision passes the check (a learned output with correct information bits but an
@@ -93,7 +94,8 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     const size_t o_app = o_hd + up((size_t)n_cw * c.hd_bytes());
     const size_t total = o_app + (app_out ? (size_t)n_cw * c.n_vn() * sizeof(float) : 0);
     void* scratch = nullptr;
-    if ((st = ctx_scratch(x, total, &scratch))) return st;
+    int st = ctx_scratch(x, total, &scratch);
+    if (st) return st;
     char* sp = static_cast<char*>(scratch);
     int* w1 = reinterpret_cast<int*>(sp);
     int* w2 = reinterpret_cast<int*>(sp + o_w2);
@@ -103,13 +105,15 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     uint8_t* hd2 = reinterpret_cast<uint8_t*>(sp + o_hd);
     float* app2 = app_out ? reinterpret_cast<float*>(sp + o_app) : nullptr;
     const hipStream_t hs = static_cast<hipStream_t>(stream);
-    if ((st = ldpc_syndrome_async(x, stream, hd, n_cw, w1))) return st;
+    // both decodes check the parity of their output after the last iteration (w1, w2: 1 = fails)
+    if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                          x->d_weights, nullptr, nullptr, w1)))
+        return st;
     // the failing codewords, compacted, so that the re-decode's workgroups are all busy ones
     if ((st = compact_failures_launch(w1, n_cw, list, cnt, hs))) return st;
     if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd2, app2, it2, nullptr,
-                          list, cnt)))
+                          list, cnt, w2)))
         return st;
-    if ((st = ldpc_syndrome_async(x, stream, hd2, n_cw, w2))) return st;
     MergeArgs m{w1, w2, hd2, app2, it2, hd, app_out, iters_out, n_cw, c.hd_bytes(), c.n_vn()};
     return fallback_merge_launch(m, hs);
 }
