@@ -11,6 +11,8 @@
 //   setup_host / setup_dev_memory /   src/setup_exp.h:13-30
 //     setup_gpu_grid / generate_new_codewords / put_h_compact_and_params_in_constant_memory
 //   setup_ldpc / ldpc_encoder_orig    src/nr_ldpc.h:14-20
+//   allocate_constant_memory /        src/ldpc_kernels.h:15-27
+//     put_params_in_constant_mem, fill_in_compact_matrices (src/setup_exp.cu:221-284)
 //   compute_noise_stddev / transmit / calculate_llr / error_check / cuda_error_check
 //                                     src/encoding_tools.h:12-28
 //   quantize / unpack_char            src/host_helpers.cpp:46-64
@@ -179,6 +181,32 @@ void put_h_compact_and_params_in_constant_memory(const ldpc_params* ldpc, const 
 template <typename T>
 void perform_gpu_ldpc_decoding(dev_mem<T>* d, host_mem<T>* h, const cuda_grid* g, int n_iterations,
                                int stream_index);
+
+// ---- the code description (src/ldpc_kernels.h:15-27, src/setup_exp.cu:221-284)
+// Compact lists of ldpc->H: h_compact1[k][i] = k-th nonzero block of block row i (ascending
+// column), h_compact2[k][j] = k-th nonzero block of block column j (ascending row), unused slots
+// {0, 0, -1, 0}.  The array extents are the caller's (the reference sizes them with its
+// H_COMPACT* macros); the row-major cores take them explicitly.
+void nr_gpu_ldpc_fill_compact(h_element* h1, int h1_deg, int h1_rows, h_element* h2, int h2_deg, int h2_cols,
+                              const ldpc_params* ldpc);
+void nr_gpu_ldpc_load_compact(const h_element* h1, int h1_deg, int h1_rows, const h_element* h2, int h2_deg,
+                              int h2_cols, const ldpc_params* ldpc);
+template <int D1, int R1, int D2, int C2>
+void fill_in_compact_matrices(h_element (&h_compact1)[D1][R1], h_element (&h_compact2)[D2][C2],
+                              const ldpc_params* ldpc) {
+    nr_gpu_ldpc_fill_compact(&h_compact1[0][0], D1, R1, &h_compact2[0][0], D2, C2, ldpc);
+}
+// Makes the code the lists describe (they must match ldpc->H exactly) the process-wide code;
+// the memory sizes are accepted for the reference's signature and not needed.
+template <int D1, int R1, int D2, int C2>
+void allocate_constant_memory(h_element (&h_compact1)[D1][R1], h_element (&h_compact2)[D2][C2],
+                              const ldpc_params* ldpc, int /*mem_size_h_compact1*/, int /*mem_size_h_compact2*/) {
+    nr_gpu_ldpc_load_compact(&h_compact1[0][0], D1, R1, &h_compact2[0][0], D2, C2, ldpc);
+}
+// Checks the kernel parameters against the process-wide code (aborts on a mismatch); the fused
+// decoder derives its launch shape itself, so threads_per_block is not used.
+void put_params_in_constant_mem(unsigned int z, unsigned int n_cw_per_mcw, unsigned int n_total_vn,
+                                unsigned int n_total_cn, unsigned int n_blk_col, unsigned int threads_per_block);
 
 // The C entry point the reference proposes for linking into OAI-style C projects
 // (Linking_With_Other_Projects.md:19-25): decode one batch of the process-wide code and geometry

@@ -363,19 +363,100 @@ void generate_new_codewords(const params* p, host_mem<T>* h, const ldpc_params* 
     }
 }
 
-void put_h_compact_and_params_in_constant_memory(const ldpc_params* ldpc, const params* /*p*/,
-                                                 const cuda_grid* g) {
+// Row list (h1[k][i]: k-th nonzero block of block row i, ascending column) and column list
+// (h2[k][j]: k-th nonzero block of block column j, ascending row) of ldpc->H; unused slots are
+// {0, 0, -1, 0}.  h1 is [h1_deg][h1_rows], h2 is [h2_deg][h2_cols], row-major.
+void nr_gpu_ldpc_fill_compact(h_element* h1, int h1_deg, int h1_rows, h_element* h2, int h2_deg, int h2_cols,
+                              const ldpc_params* ldpc) {
+    const int R = ldpc->H_rows, C = ldpc->H_cols;
+    if (h1_rows < R || h2_cols < C) config_error("compact matrices smaller than the base matrix");
+    for (int k = 0; k < h1_deg; k++)
+        for (int i = 0; i < h1_rows; i++) h1[k * h1_rows + i] = {0, 0, -1, 0};
+    for (int k = 0; k < h2_deg; k++)
+        for (int j = 0; j < h2_cols; j++) h2[k * h2_cols + j] = {0, 0, -1, 0};
+    for (int i = 0; i < R; i++) {
+        int k = 0;
+        for (int j = 0; j < C; j++) {
+            const int v = ldpc->H[i * C + j];
+            if (v < 0) continue;
+            if (k >= h1_deg) config_error("row degree exceeds the compact row list");
+            h1[k++ * h1_rows + i] = {(char)i, (char)j, (short)v, 1};
+        }
+    }
+    for (int j = 0; j < C; j++) {
+        int k = 0;
+        for (int i = 0; i < R; i++) {
+            const int v = ldpc->H[i * C + j];
+            if (v < 0) continue;
+            if (k >= h2_deg) config_error("column degree exceeds the compact column list");
+            h2[k++ * h2_cols + j] = {(char)i, (char)j, (short)v, 1};
+        }
+    }
+}
+
+// The process-wide code from its compact lists, which must describe ldpc->H exactly.
+void nr_gpu_ldpc_load_compact(const h_element* h1, int h1_deg, int h1_rows, const h_element* h2, int h2_deg,
+                              int h2_cols, const ldpc_params* ldpc) {
+    const int R = ldpc->H_rows, C = ldpc->H_cols;
+    if (h1_rows < R || h2_cols < C) config_error("compact matrices smaller than the base matrix");
+    auto same = [](const h_element& e, int i, int j, int v) {
+        return e.valid == 1 && e.x == (char)i && e.y == (char)j && e.value == (short)v;
+    };
+    for (int i = 0; i < R; i++) {
+        int k = 0;
+        for (int j = 0; j < C; j++) {
+            const int v = ldpc->H[i * C + j];
+            if (v < 0) continue;
+            if (k >= h1_deg || !same(h1[k * h1_rows + i], i, j, v))
+                config_error("h_compact1 does not describe ldpc->H");
+            k++;
+        }
+        if (k < h1_deg && h1[k * h1_rows + i].valid) config_error("h_compact1 has blocks ldpc->H does not have");
+    }
+    for (int j = 0; j < C; j++) {
+        int k = 0;
+        for (int i = 0; i < R; i++) {
+            const int v = ldpc->H[i * C + j];
+            if (v < 0) continue;
+            if (k >= h2_deg || !same(h2[k * h2_cols + j], i, j, v))
+                config_error("h_compact2 does not describe ldpc->H");
+            k++;
+        }
+        if (k < h2_deg && h2[k * h2_cols + j].valid) config_error("h_compact2 has blocks ldpc->H does not have");
+    }
     if (g_ctx) {
         ldpc_ctx_destroy(g_ctx);
         g_ctx = nullptr;
     }
     free_centry_buffers();
-    g_n_cw = g ? (int)(g->cnp_blocks.y * g->cnp_threads.y) : 0;
+    g_n_cw = 0;
     g_n_iter = ldpc->n_iterations;
     g_n_vn = ldpc->N_before_puncture;
     int dev = 0;
     NRLDPC_CHECK_HIP(hipGetDevice(&dev));
     NRLDPC_CHECK(ldpc_ctx_create(&g_ctx, ldpc->bg_index, ldpc->Z, ldpc->H_rows, dev));
+}
+
+void put_params_in_constant_mem(unsigned int z, unsigned int n_cw_per_mcw, unsigned int n_total_vn,
+                                unsigned int n_total_cn, unsigned int n_blk_col, unsigned int threads_per_block) {
+    if (!g_ctx) config_error("put_params_in_constant_mem before allocate_constant_memory");
+    ldpc_code_info info;
+    NRLDPC_CHECK(ldpc_ctx_info(g_ctx, &info));
+    if ((int)z != info.Z || (int)n_total_vn != info.n_vn || (int)n_total_cn != info.rows * info.Z ||
+        (int)n_blk_col != info.cols || n_cw_per_mcw < 1)
+        config_error("kernel parameters do not match the code in constant memory");
+    (void)threads_per_block;  // the fused launch chooses its own workgroup shape
+}
+
+void put_h_compact_and_params_in_constant_memory(const ldpc_params* ldpc, const params* p, const cuda_grid* g) {
+    // the reference's sequence: compact lists of H, upload, then the kernel parameters
+    const int d1 = ldpc->n_nz_in_row, d2 = ldpc->n_nz_in_col;
+    std::vector<h_element> h1((size_t)d1 * ldpc->H_rows), h2((size_t)d2 * ldpc->H_cols);
+    nr_gpu_ldpc_fill_compact(h1.data(), d1, ldpc->H_rows, h2.data(), d2, ldpc->H_cols, ldpc);
+    nr_gpu_ldpc_load_compact(h1.data(), d1, ldpc->H_rows, h2.data(), d2, ldpc->H_cols, ldpc);
+    put_params_in_constant_mem(ldpc->Z, p ? p->n_cw_per_mcw : 1, ldpc->N_before_puncture, ldpc->H_rows * ldpc->Z,
+                               ldpc->H_cols, g ? g->cnp_threads_per_block : 0);
+    g_n_cw = g ? (int)(g->cnp_blocks.y * g->cnp_threads.y) : 0;
 }
 
 template <typename T>

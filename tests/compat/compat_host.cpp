@@ -11,9 +11,19 @@ static void print_ints(const char* name, const int* v, int n, bool last = false)
     printf("]%s\n", last ? "" : ",");
 }
 
-int main() {
-    printf("{\n");
+static h_element g_h1[24][46];  // wider than the largest row degree (19)
+static h_element g_h2[32][68];  // wider than the largest column degree (30)
+
+int main(int argc, char** argv) {
     const ldpc_params a = setup_ldpc(8424, 1.0f / 3.0f, 5);
+    fill_in_compact_matrices(g_h1, g_h2, &a);
+    if (argc > 1) {
+        // a row list that no longer describes H is refused before any device work
+        g_h1[3][5].value = (short)(g_h1[3][5].value + 1);
+        allocate_constant_memory(g_h1, g_h2, &a, sizeof g_h1, sizeof g_h2);
+        return 0;
+    }
+    printf("{\n");
     const int fa[] = {a.bg_index, a.Z, a.K, a.N, a.H_rows, a.H_cols, a.N_before_puncture, a.n_nz_in_row,
                       a.n_nz_in_col, a.n_iterations, a.h_element_count_per_row[0], a.h_element_count_per_col[0]};
     print_ints("setup_bg1", fa, 12);
@@ -72,6 +82,23 @@ int main() {
     printf("\"tx_recv\": [");
     for (int i = 0; i < nb; i++) printf("%s%.9g", i ? "," : "", rv[i]);
     printf("],\n");
+    // compact lists: valid entries, then the first row's list and the first column's list
+    int n1 = 0, n2 = 0;
+    for (auto& row : g_h1)
+        for (auto& e : row) n1 += e.valid;
+    for (auto& row : g_h2)
+        for (auto& e : row) n2 += e.valid;
+    int r0[3 * 19], c0[3 * 30];
+    for (int k = 0; k < 19; k++) {
+        r0[3 * k] = g_h1[k][0].x; r0[3 * k + 1] = g_h1[k][0].y; r0[3 * k + 2] = g_h1[k][0].value;
+    }
+    for (int k = 0; k < 30; k++) {
+        c0[3 * k] = g_h2[k][0].x; c0[3 * k + 1] = g_h2[k][0].y; c0[3 * k + 2] = g_h2[k][0].value;
+    }
+    const int counts[] = {n1, n2, g_h1[19][0].value, g_h1[19][0].valid, g_h2[30][0].value};
+    print_ints("compact_counts", counts, 5);
+    print_ints("compact_row0", r0, 3 * 19);
+    print_ints("compact_col0", c0, 3 * 30);
     const short* g = choose_generator_matrix(1, 384);
     printf("\"gen_first\": [%d, %d, %d]\n", g[0], g[1], g[2]);
     printf("}\n");

@@ -23,6 +23,35 @@ def host_out(tmp_path_factory):
     return json.loads(r.stdout)
 
 
+def test_compact_lists(host_out, nr):
+    """fill_in_compact_matrices: one entry per nonzero block, row lists in ascending column,
+    column lists in ascending row, unused slots {0, 0, -1, 0}."""
+    H = np.array(host_out["H_bg1_row0"])
+    n1, n2, pad_v, pad_valid, pad2_v = host_out["compact_counts"]
+    assert n1 == n2 == 316
+    assert pad_v == -1 and pad_valid == 0 and pad2_v == -1
+    row0 = np.array(host_out["compact_row0"]).reshape(19, 3)
+    cols = np.nonzero(H >= 0)[0]
+    assert (row0[:, 0] == 0).all() and list(row0[:, 1]) == list(cols) and list(row0[:, 2]) == list(H[cols])
+    col0 = np.array(host_out["compact_col0"]).reshape(30, 3)
+    base = nr.base_matrix(nr.BG1, 1)  # Z = 384 is in lifting set 1
+    rows = np.nonzero(base[:, 0] >= 0)[0]
+    assert (col0[:, 1] == 0).all()
+    assert list(col0[:, 0]) == list(rows) and list(col0[:, 2]) == list(base[rows, 0])
+
+
+def test_allocate_constant_memory_refuses_a_wrong_list(tmp_path):
+    """A row list that does not describe ldpc->H is refused (message, abort) before any device
+    call, so this runs without a GPU."""
+    exe = str(tmp_path / "compat_host")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", f"-I{ROOT}/include",
+                    f"{ROOT}/tests/compat/compat_host.cpp", f"-L{LIBDIR}", "-lnrldpc", f"-Wl,-rpath,{LIBDIR}",
+                    "-o", exe], check=True, capture_output=True)
+    r = subprocess.run([exe, "corrupt"], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "h_compact1 does not describe ldpc->H" in r.stderr
+
+
 def test_setup_ldpc(host_out):
     # BG1 Z=384: K=8448, N=25344, 46 x 68, 26112 decoder columns, row degree 19, column degree 30
     assert host_out["setup_bg1"] == [1, 384, 8448, 25344, 46, 68, 26112, 19, 30, 5, 19, 30]
