@@ -306,8 +306,20 @@ def bench_stream(torch, nr, args):
     ebn0_points = (1.0, 1.5, 2.0)
     batches = [(c, int(rng.integers(c // 2 + 1, c + 1)), ebn0_points[i % 3]) for i, c in enumerate(classes * 3)]
     batches += [(8192, 8192, e) for e in ebn0_points for _ in range(2)]  # FER statistics
-    stats = {mode: {"lat": [], "info": 0, "time_ms": 0.0, "fer": {e: [0, 0] for e in ebn0_points}}
+    stats = {mode: {"lat": [], "info": 0, "bytes": 0, "time_ms": 0.0, "fer": {e: [0, 0] for e in ebn0_points}}
              for mode in modes}
+    # the bus these batches cross: best of three pinned host-to-device copies of 256 MiB
+    n_cal = 64 * 1024 * 1024
+    cal = []
+    with torch.cuda.stream(stream):
+        for _ in range(3):
+            c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            c0.record(stream)
+            d_llr[:n_cal].copy_(h_llr[:n_cal], non_blocking=True)
+            c1.record(stream)
+            stream.synchronize()
+            cal.append(c0.elapsed_time(c1))
+    h2d_gbs = 4 * n_cal / (min(cal) * 1e6)
     contexts = {}
     for bi, (c, n, e) in enumerate(batches):
         kind = np.where(rng.random(n) < 0.5, nr.BG1, nr.BG2)
@@ -331,6 +343,7 @@ def bench_stream(torch, nr, args):
             st = stats[mode]
             st["lat"].append(ms)
             st["info"] += int(b["info_bits"].sum())
+            st["bytes"] += 4 * b["llr_floats"] + b["hd_total"]
             st["time_ms"] += ms
             st["fer"][e][0] += fe
             st["fer"][e][1] += n
@@ -338,9 +351,14 @@ def bench_stream(torch, nr, args):
                        "batches of 1..8192 codewords (sizes in (2^k/2, 2^k], 3 per power of two, plus 2 x 8192 "
                        "per Eb/N0 point), Eb/N0 1.0 / 1.5 / 2.0 dB, T=10 with early exit; each batch is one "
                        "replayed HIP graph: LLR upload, plan upload, persistent decode, hard-decision download",
-           "learned_weights": sorted(f"{k}/{z}" for k, z in weights)}
+           "learned_weights": sorted(f"{k}/{z}" for k, z in weights),
+           "pcie": {"h2d_gbs": round(h2d_gbs, 2),
+                    "note": "best pinned host-to-device copy of 256 MiB; every batch moves 4 B per LLR in "
+                            "and its packed decisions out, so info Gbps <= info bits / (bytes / h2d_gbs)"}}
     for mode, st in stats.items():
         out[mode] = {"info_gbps": round(st["info"] / (st["time_ms"] * 1e6), 3),
+                     "bytes_moved": st["bytes"],
+                     "copy_bound_info_gbps": round(st["info"] * h2d_gbs / st["bytes"], 3),
                      "latency_ms": {"p50": round(statistics.median(st["lat"]), 3),
                                     "p99": round(pct(st["lat"], 0.99), 3),
                                     "kind": "device time per batch incl. host copies"},
