@@ -9,8 +9,9 @@ Also measured, each in its own object of the same JSON line:
   * ``mixed_batch``: 16384 codewords of random base graph / lifting size / rate / Eb/N0 in
     one persistent launch, at most 20 iterations with early exit (autotuned tiles);
   * ``combined_stream``: BG1 Z=384 and BG2 Z=208 codewords at mixed rates in batches of
-    1..8192, each batch a replayed HIP graph (LLR upload, plan upload, decode, hard-decision
-    download), plain min-sum against learned weights with plain fallback;
+    1..8192, each batch an LLR upload, a replayed HIP graph (plan upload, decode) and a
+    hard-decision download, plain min-sum against learned weights with plain fallback; batch
+    by batch (latency) and pipelined over two streams (throughput);
   * ``stream_overlap``: the same traffic as batches of 16 / 256 / 2048 codewords over 1, 2 and 4
     streams round-robin (host planning and PCIe copies overlapped with decoding);
   * ``cpu_baseline``: the C oracle (same schedule) on this host's cores.
@@ -321,14 +322,25 @@ def bench_stream(torch, nr, args):
             cal.append(c0.elapsed_time(c1))
     h2d_gbs = 4 * n_cal / (min(cal) * 1e6)
     contexts = {}
+    drawn = []
     for bi, (c, n, e) in enumerate(batches):
         kind = np.where(rng.random(n) < 0.5, nr.BG1, nr.BG2)
         Zs = np.where(kind == nr.BG1, 384, 208)
         r_nom = rng.uniform(0.2, 8 / 9, n)
         mr = np.where(kind == nr.BG1, mx.rows_for_rate(nr.BG1, r_nom), mx.rows_for_rate(nr.BG2, r_nom))
-        b = mx.describe(kind, Zs, mr, ebn0=np.full(n, e))
+        drawn.append(mx.describe(kind, Zs, mr, ebn0=np.full(n, e)))
+    # every batch's LLRs in one pinned buffer, and room for every batch's decisions, for the
+    # pipelined pass below
+    l_off = np.concatenate([[0], np.cumsum([b["llr_floats"] for b in drawn])])
+    h_off = np.concatenate([[0], np.cumsum([b["hd_total"] for b in drawn])])
+    all_llr = torch.empty(int(l_off[-1]), dtype=torch.float32).pin_memory()
+    all_hd = {mode: torch.zeros(int(h_off[-1]), dtype=torch.uint8).pin_memory() for mode in modes}
+    want_hd = {mode: torch.zeros(int(h_off[-1]), dtype=torch.uint8) for mode in modes}
+    for bi, (c, n, e) in enumerate(batches):
+        b = drawn[bi]
         llr, bits = mx.make_channel(torch, b, seed=1000 + bi, contexts=contexts)
         h_llr[:b["llr_floats"]].copy_(llr.cpu())
+        all_llr[int(l_off[bi]):int(l_off[bi + 1])].copy_(h_llr[:b["llr_floats"]])
         for mode in modes:
             m = objs[(mode, c)]
             m.plan(b["desc"])
@@ -338,6 +350,7 @@ def bench_stream(torch, nr, args):
             e1.record(stream)
             stream.synchronize()
             ms = e0.elapsed_time(e1)
+            want_hd[mode][int(h_off[bi]):int(h_off[bi + 1])].copy_(h_hd[:b["hd_total"]])
             hd = h_hd[:b["hd_total"]].cuda()
             _, fe, _ = mx.info_errors(torch, b, hd, bits)
             st = stats[mode]
@@ -355,8 +368,46 @@ def bench_stream(torch, nr, args):
            "pcie": {"h2d_gbs": round(h2d_gbs, 2),
                     "note": "best pinned host-to-device copy of 256 MiB; every batch moves 4 B per LLR in "
                             "and its packed decisions out, so info Gbps <= info bits / (bytes / h2d_gbs)"}}
+    # The same sequence pipelined over two streams: batch i + 1 is planned, uploaded and
+    # decoded while batch i is decoded or downloaded.  Each stream has its own mixed object
+    # (graph: plan upload + decode) and device buffers; uploads and downloads are stream copies
+    # from and to the pinned buffers holding every batch.
+    n_pipe = 2
+    pstreams = hip_streams(torch, n_pipe)
+    for mode, flags in modes.items():
+        slots = []
+        for k in range(n_pipe):
+            m = nr.Mixed(8192)
+            if mode == "learned":
+                for (kind, z), w in weights.items():
+                    m.set_weights(kind, z, w)
+            dl = torch.zeros(max_f, dtype=torch.float32, device="cuda")
+            dh = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
+            m.plan(drawn[0]["desc"])
+            m.capture(pstreams[k].cuda_stream, dl.data_ptr(), T, dh.data_ptr(), 0, 0, flags=flags)
+            slots.append((m, dl, dh))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for bi, b in enumerate(drawn):
+            m, dl, dh = slots[bi % n_pipe]
+            ps = pstreams[bi % n_pipe]
+            m.plan(b["desc"])  # waits until this stream's previous batch has left its buffers
+            with torch.cuda.stream(ps):
+                dl[:b["llr_floats"]].copy_(all_llr[int(l_off[bi]):int(l_off[bi + 1])], non_blocking=True)
+            m.replay(ps.cuda_stream)
+            with torch.cuda.stream(ps):
+                all_hd[mode][int(h_off[bi]):int(h_off[bi + 1])].copy_(dh[:b["hd_total"]], non_blocking=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if not torch.equal(all_hd[mode], want_hd[mode]):
+            raise RuntimeError(f"pipelined {mode} stream decisions differ from the serial pass")
+        stats[mode]["pipelined"] = {"streams": n_pipe, "info_gbps": round(stats[mode]["info"] / (wall * 1e9), 3),
+                                    "wall_ms": round(wall * 1e3, 2)}
+        for m, _, _ in slots:
+            m.close()
     for mode, st in stats.items():
         out[mode] = {"info_gbps": round(st["info"] / (st["time_ms"] * 1e6), 3),
+                     "pipelined": st["pipelined"],
                      "bytes_moved": st["bytes"],
                      "copy_bound_info_gbps": round(st["info"] * h2d_gbs / st["bytes"], 3),
                      "latency_ms": {"p50": round(statistics.median(st["lat"]), 3),
@@ -369,25 +420,20 @@ def bench_stream(torch, nr, args):
 
 def hip_streams(torch, n):
     """n fresh HIP streams created back to back (the runtime hands out its hardware queues
-    round-robin, so consecutive streams do not share one), wrapped for torch, and a function
-    that destroys them."""
+    round-robin, so consecutive streams do not share one), wrapped for torch.  They live until
+    the process ends: torch's pinned-memory allocator records events on every stream a pinned
+    buffer was copied on, also when the buffer is freed later."""
     import ctypes
     import importlib.util
     spec = importlib.util.find_spec("torch")
     hip = ctypes.CDLL(os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so"))
-    handles = []
+    out = []
     for _ in range(n):
         h = ctypes.c_void_p()
         if hip.hipStreamCreate(ctypes.byref(h)) != 0:
             raise RuntimeError("hipStreamCreate failed")
-        handles.append(h)
-
-    def destroy():
-        torch.cuda.synchronize()
-        for h in handles:
-            hip.hipStreamDestroy(h)
-
-    return [torch.cuda.ExternalStream(h.value) for h in handles], destroy
+        out.append(torch.cuda.ExternalStream(h.value))
+    return out
 
 
 def bench_pipeline(torch, nr, args):
@@ -401,7 +447,7 @@ def bench_pipeline(torch, nr, args):
     T = 10
     S_MAX = 4
     rng = np.random.default_rng(7)
-    streams, destroy = hip_streams(torch, S_MAX)
+    streams = hip_streams(torch, S_MAX)
     contexts = {}
     out = {"workload": "BG1 Z=384 / BG2 Z=208 (50/50) at rates U[1/5, 8/9], Eb/N0 1.5 dB, T=10 with "
                        "early exit; per batch: plan on the host, then one replayed HIP graph (pinned "
@@ -461,7 +507,6 @@ def bench_pipeline(torch, nr, args):
         out[f"batch_{c}"] = row
         for sl in slots:
             sl["m"].close()
-    destroy()
     return out
 
 

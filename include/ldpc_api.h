@@ -241,9 +241,11 @@ int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order);
 int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags,
                             uint8_t* hd, float* app_out, int* iters_out);
 
-/* Build a HIP graph owned by p: an optional host->device copy of llr_host into llr (at most
- * llr_floats floats), the plan upload, the decode (as ldpc_mixed_decode_async) and an optional
- * device->host copy of hd into hd_host (at most hd_bytes).  Host buffers must be pinned
+/* Build a HIP graph owned by p: the plan upload and the decode (as ldpc_mixed_decode_async).
+ * Each replay enqueues, on its stream, an optional host->device copy of llr_host into llr (at
+ * most llr_floats floats), the graph, and an optional device->host copy of hd into hd_host (at
+ * most hd_bytes); the two batch copies are stream copies (DMA engines) so that replays on
+ * different streams overlap them with each other's decodes.  Host buffers must be pinned
  * (hipHostMalloc).  Each replay decodes the plan current at that time, into the same buffers,
  * and cuts the copies to the LLR floats and hard-decision bytes that plan addresses.  `stream`
  * is unused (the graph runs on the replay's stream). */

@@ -72,7 +72,7 @@ struct ldpc_mixed {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     // memcpy nodes of the graph (null when absent) and what they copy at most
-    hipGraphNode_t n_llr = nullptr, n_head = nullptr, n_body = nullptr, n_hd = nullptr;
+    hipGraphNode_t n_head = nullptr, n_body = nullptr;
     const float* g_llr_host = nullptr;
     float* g_llr = nullptr;
     uint8_t* g_hd = nullptr;
@@ -396,33 +396,30 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
     if (st) return st;
     if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
     if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
-    p->n_llr = p->n_head = p->n_body = p->n_hd = nullptr;
+    p->n_head = p->n_body = nullptr;
     hipGraph_t g = nullptr;
     NRLDPC_HIP(hipGraphCreate(&g, 0));
     p->graph = g;
-    // chain: [LLR upload] -> plan header + tiles -> codeword table -> decode -> [HD download];
-    // every copy is created at its largest size and cut to the current plan at each replay
-    hipGraphNode_t prev = nullptr, kern = nullptr;
-    if (llr_host && llr_floats) {
-        NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_llr, g, nullptr, 0, const_cast<float*>(llr), llr_host,
-                                           llr_floats * sizeof(float), hipMemcpyHostToDevice));
-        prev = p->n_llr;
-    }
-    NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_head, g, prev ? &prev : nullptr, prev ? 1 : 0, p->d_plan, p->h_plan,
+    // graph: plan header + tiles -> codeword table -> decode; every copy is created at its
+    // largest size and cut to the current plan at each replay.  The LLR upload and the
+    // hard-decision download are stream copies around the graph launch (replay): graph copy
+    // nodes of batch size do not overlap kernels of other streams, stream copies go to the DMA
+    // engines and do.
+    hipGraphNode_t kern = nullptr;
+    NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_head, g, nullptr, 0, p->d_plan, p->h_plan,
                                        p->cws_off, hipMemcpyHostToDevice));
     NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_body, g, &p->n_head, 1, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
                                        (size_t)p->max_cw * sizeof(MixedCw), hipMemcpyHostToDevice));
     if ((st = mixed_graph_node((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, g,
                                &p->n_body, 1, &kern)))
         return st;
-    if (hd_host && hd_bytes)
-        NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_hd, g, &kern, 1, hd_host, hd, hd_bytes, hipMemcpyDeviceToHost));
+    (void)kern;
     p->g_llr_host = llr_host;
     p->g_llr = const_cast<float*>(llr);
-    p->g_llr_floats = p->n_llr ? llr_floats : 0;
+    p->g_llr_floats = llr_host ? llr_floats : 0;
     p->g_hd = hd;
     p->g_hd_host = hd_host;
-    p->g_hd_bytes = p->n_hd ? hd_bytes : 0;
+    p->g_hd_bytes = hd_host ? hd_bytes : 0;
     NRLDPC_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     return LDPC_OK;
 }
@@ -435,20 +432,16 @@ int ldpc_mixed_replay(ldpc_mixed* p, void* stream) {
     NRLDPC_HIP(hipSetDevice(p->device));
     // The previous replay has finished (planning waited for it), so the copies can be resized
     // to what the current plan reads and writes.
-    const size_t lf = (size_t)std::max<long long>(1, std::min<long long>((long long)p->g_llr_floats, p->plan_llr_extent));
-    const size_t hb = (size_t)std::max<long long>(1, std::min<long long>((long long)p->g_hd_bytes, p->plan_hd_extent));
-    if (p->n_llr)
-        NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_llr, p->g_llr, p->g_llr_host, lf * sizeof(float),
-                                                     hipMemcpyHostToDevice));
+    const size_t lf = (size_t)std::min<long long>((long long)p->g_llr_floats, p->plan_llr_extent);
+    const size_t hb = (size_t)std::min<long long>((long long)p->g_hd_bytes, p->plan_hd_extent);
     NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_head, p->d_plan, p->h_plan, plan_head_bytes(p),
                                                  hipMemcpyHostToDevice));
     NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_body, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
                                                  plan_body_bytes(p), hipMemcpyHostToDevice));
-    if (p->n_hd)
-        NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_hd, p->g_hd_host, p->g_hd, hb,
-                                                     hipMemcpyDeviceToHost));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    if (lf) NRLDPC_HIP(hipMemcpyAsync(p->g_llr, p->g_llr_host, lf * sizeof(float), hipMemcpyHostToDevice, s));
     NRLDPC_HIP(hipGraphLaunch(p->exec, s));
+    if (hb) NRLDPC_HIP(hipMemcpyAsync(p->g_hd_host, p->g_hd, hb, hipMemcpyDeviceToHost, s));
     NRLDPC_HIP(hipEventRecord(p->uploaded, s));
     p->upload_pending = true;
     return LDPC_OK;
