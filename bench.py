@@ -180,6 +180,21 @@ def bench_headline(torch, nr, args):
     hev[1].record(stream)
     torch.cuda.synchronize()
     half_ms = hev[0].elapsed_time(hev[1]) / args.steps
+    # the reference's CHAR_PRC schedule (its default build) on the same batch: LLRs quantised as
+    # quantize(0.5, x, 8) does (1 byte per LLR)
+    llr_q = torch.clamp(torch.floor(llr * 2.0), -128, 127).to(torch.int8)
+    code.decode_char(llr_q.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    cev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    cev[0].record(stream)
+    for _ in range(args.steps):
+        code.decode_char(llr_q.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+    cev[1].record(stream)
+    torch.cuda.synchronize()
+    char_ms = cev[0].elapsed_time(cev[1]) / args.steps
+    code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, counts.data_ptr(), stream=s)
+    torch.cuda.synchronize()
+    char_fe = int(counts.tolist()[1]) - frame_err
     res = {
         "value": round(gbps, 4), "ms_per_step": round(ms, 4),
         "cw_per_block": code.cw_per_block,
@@ -188,6 +203,10 @@ def bench_headline(torch, nr, args):
         "fer": round(frame_err / BATCH, 6), "ber": bit_err / (BATCH * code.K),
         "half_prc": {"info_gbps": round(BATCH * code.K / (half_ms * 1e6), 4), "ms_per_step": round(half_ms, 4),
                      "note": "the reference's HALF_PRC schedule: binary16 LLRs, R, dt and inter-iteration APP"},
+        "char_prc": {"info_gbps": round(BATCH * code.K / (char_ms * 1e6), 4), "ms_per_step": round(char_ms, 4),
+                     "fer": round(char_fe / BATCH, 6),
+                     "note": "the reference's CHAR_PRC schedule (its default build): signed-char LLRs "
+                             "(quantize(0.5, x, 8)), R, dt and inter-iteration APP"},
         "roofline": {
             "bound": "hbm",
             "achieved": round(hbm_gbs, 1),
@@ -626,6 +645,7 @@ def main():
         "ber": head["ber"],
         "roofline": head["roofline"],
         "half_prc": head["half_prc"],
+        "char_prc": head["char_prc"],
     }
     op = bench_operating_point(torch, nr, args)
     if op:

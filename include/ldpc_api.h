@@ -88,6 +88,16 @@ int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, i
  * single-precision posterior). */
 int ldpc_decode_half_async(ldpc_ctx* ctx, void* stream, const uint16_t* llr_half, int n_cw, int n_iter,
                            unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+/* The reference's CHAR_PRC build (its default, src/globals.h:12-13): signed-char channel LLRs
+ * ([n_cw][n_vn], as quantize(0.5, x, 8) makes them, src/host_helpers.cpp:46-58, used as the LLR
+ * values), R and dt stored as signed chars and the posterior between iterations clamped to
+ * [-128, 127] (src/ldpc_kernels.cu CHAR_PRC branches: :173-175, :258-264, :304-306, :387-390), a
+ * saturated posterior (127 or -128) entering the check node without R_old (:260-262).  float ->
+ * char conversions truncate toward zero; where the value lies outside [-128, 127], which C++
+ * leaves undefined, they saturate.  Outputs as for ldpc_decode_async (app_out: the final
+ * posterior before clamping). */
+int ldpc_decode_char_async(ldpc_ctx* ctx, void* stream, const int8_t* llr_i8, int n_cw, int n_iter,
+                           unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 /* int8 LLRs (as the reference's quantize(D, x, 8) makes them, src/host_helpers.cpp:46-58) ->
  * float LLRs out[i] = in[i] * scale (scale = D), n values. */
 int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float scale, float* out);

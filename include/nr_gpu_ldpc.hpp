@@ -212,9 +212,10 @@ void put_params_in_constant_mem(unsigned int z, unsigned int n_cw_per_mcw, unsig
 // (Linking_With_Other_Projects.md:19-25): decode one batch of the process-wide code and geometry
 // (set by put_h_compact_and_params_in_constant_memory: n_mcw * n_cw_per_mcw codewords,
 // ldpc_params::n_iterations iterations).  inputs: int8 LLRs [n_cw][N_before_puncture] in host
-// memory, quantised as quantize(0.5, llr, 8) does; they are decoded as llr = 0.5 * q in single
-// precision.  outputs: hard decisions packed over the concatenated [n_cw][N_before_puncture]
-// bit stream (bit g of byte g/8), ceil(n_cw * N_before_puncture / 8) bytes.  Synchronous.
+// memory, quantised as quantize(0.5, llr, 8) does; they are decoded with the reference's
+// CHAR_PRC schedule, its default build (ldpc_decode_char_async).  outputs: hard decisions packed
+// over the concatenated [n_cw][N_before_puncture] bit stream (bit g of byte g/8),
+// ceil(n_cw * N_before_puncture / 8) bytes.  Synchronous.
 extern "C" void ldpc_gpu_decoder(char* inputs, char* outputs);
 
 // The process-wide decoder context created by put_h_compact_and_params_in_constant_memory

@@ -81,6 +81,7 @@ _sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_decode_half_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_decode_char_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_llr_to_half_async", ctypes.c_int, _P, _P, ctypes.c_size_t, _P)
 _sig("ldpc_base_matrix", ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_int)
 _sig("ldpc_setup_params", ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_int_p)
@@ -223,6 +224,11 @@ class Code:
         """The reference's HALF_PRC schedule; llr_half_ptr: device binary16 [n_cw][n_vn]."""
         _check(lib.ldpc_decode_half_async(self._h, stream, llr_half_ptr, n_cw, n_iter, flags, hd_ptr,
                                           app_ptr or None, iters_ptr or None), "ldpc_decode_half_async")
+
+    def decode_char(self, llr_i8_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        """The reference's CHAR_PRC schedule; llr_i8_ptr: device int8 [n_cw][n_vn]."""
+        _check(lib.ldpc_decode_char_async(self._h, stream, llr_i8_ptr, n_cw, n_iter, flags, hd_ptr,
+                                          app_ptr or None, iters_ptr or None), "ldpc_decode_char_async")
 
     def syndrome(self, hd_ptr, n_cw, weight_ptr, stream=0):
         _check(lib.ldpc_syndrome_async(self._h, stream, hd_ptr, n_cw, weight_ptr), "ldpc_syndrome_async")

@@ -399,6 +399,13 @@ int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float sca
     return from_int8_launch(in, n, scale, out, static_cast<hipStream_t>(stream));
 }
 
+int ldpc_decode_char_async(ldpc_ctx* x, void* stream, const int8_t* llr, int n_cw, int n_iter, unsigned flags,
+                           uint8_t* hd, float* app_out, int* iters_out) {
+    if (!x || !llr || !hd || n_cw < 0 || n_iter < 1) return LDPC_EINVAL;
+    if (n_cw == 0) return LDPC_OK;
+    return decode_char(x, stream, llr, n_cw, n_iter, flags, hd, app_out, iters_out);
+}
+
 int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out) {
     if ((!llr || !out) && n) return LDPC_EINVAL;
     return to_half_launch(llr, n, out, static_cast<hipStream_t>(stream));

@@ -24,7 +24,6 @@ int g_n_cw = 0, g_n_iter = 0, g_n_vn = 0;  // batch geometry of that code (ldpc_
 // Device buffers of ldpc_gpu_decoder, sized for the process-wide geometry.
 struct CEntryBuffers {
     int8_t* in = nullptr;
-    float* llr = nullptr;
     uint8_t* hd = nullptr;
     uint8_t* packed = nullptr;
     hipStream_t stream = nullptr;
@@ -34,7 +33,6 @@ CEntryBuffers g_cbuf;
 
 void free_centry_buffers() {
     if (g_cbuf.in) (void)hipFree(g_cbuf.in);
-    if (g_cbuf.llr) (void)hipFree(g_cbuf.llr);
     if (g_cbuf.hd) (void)hipFree(g_cbuf.hd);
     if (g_cbuf.packed) (void)hipFree(g_cbuf.packed);
     if (g_cbuf.stream) (void)hipStreamDestroy(g_cbuf.stream);
@@ -484,17 +482,16 @@ extern "C" void ldpc_gpu_decoder(char* inputs, char* outputs) {
         free_centry_buffers();
         NRLDPC_CHECK_HIP(hipStreamCreate(&g_cbuf.stream));
         NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.in), n));
-        NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.llr), n * sizeof(float)));
         NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.hd), (size_t)g_n_cw * hb));
         NRLDPC_CHECK_HIP(hipMalloc(reinterpret_cast<void**>(&g_cbuf.packed), packed));
         g_cbuf.n = n;
     }
     const hipStream_t s = g_cbuf.stream;
     NRLDPC_CHECK_HIP(hipMemcpyAsync(g_cbuf.in, inputs, n, hipMemcpyHostToDevice, s));
-    NRLDPC_CHECK(ldpc_llr_from_int8_async(s, g_cbuf.in, n, 0.5f, g_cbuf.llr));
     const bool repack = g_n_vn % 8 != 0;
     uint8_t* hd = repack ? g_cbuf.hd : g_cbuf.packed;
-    NRLDPC_CHECK(ldpc_decode_async(g_ctx, s, g_cbuf.llr, g_n_cw, g_n_iter, 0, hd, nullptr, nullptr));
+    // char LLRs: the CHAR_PRC schedule, the reference's default build for this entry point
+    NRLDPC_CHECK(ldpc_decode_char_async(g_ctx, s, g_cbuf.in, g_n_cw, g_n_iter, 0, hd, nullptr, nullptr));
     if (repack) NRLDPC_CHECK(ldpc_hd_to_stream_async(g_ctx, s, hd, g_n_cw, g_cbuf.packed));
     NRLDPC_CHECK_HIP(hipMemcpyAsync(outputs, g_cbuf.packed, packed, hipMemcpyDeviceToHost, s));
     NRLDPC_CHECK_HIP(hipStreamSynchronize(s));

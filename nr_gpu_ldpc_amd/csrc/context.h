@@ -47,5 +47,7 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
                 const int* list_count, int* fail_out = nullptr);
 int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,
                 float* app_out, int* iters_out);
+int decode_char(ldpc_ctx* x, void* stream, const int8_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,
+                float* app_out, int* iters_out);
 
 }  // namespace nrldpc

@@ -60,4 +60,28 @@ int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_
     return decoder_launch_half(c.kind, a, static_cast<hipStream_t>(stream));
 }
 
+int decode_char(ldpc_ctx* x, void* stream, const int8_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,
+                float* app_out, int* iters_out) {
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    const CodeDesc& c = x->code;
+    DecodeArgs a{};
+    a.llr_i8 = llr;
+    a.hd = hd;
+    a.app_out = app_out;
+    a.iters_out = iters_out;
+    a.etab = x->d_etab;
+    a.n_cw = n_cw;
+    a.Z = c.Z;
+    a.m_rows = c.rows;
+    a.n_vn = c.n_vn();
+    a.hd_bytes = c.hd_bytes();
+    a.n_iter = n_iter;
+    a.cpb = x->cpb;
+    a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
+    a.lds_cw_bytes = (int)decoder_lds_bytes_per_cw(c.kind, c.Z);
+    a.nnz = nnz_full(c.kind);
+    return decoder_launch_char(c.kind, a, static_cast<hipStream_t>(stream));
+}
+
 }  // namespace nrldpc

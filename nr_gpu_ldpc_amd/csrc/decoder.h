@@ -41,6 +41,7 @@ struct DecodeArgs {
     const int* list;       // or null: decode only codewords list[0 .. *list_count - 1]
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
+    const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -92,6 +93,8 @@ int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
 // The reference's HALF_PRC schedule (binary16 LLRs in a.llr_half).
 int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s);
+// The reference's CHAR_PRC schedule (signed-char LLRs in a.llr_i8).
+int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s);
 // Learned forward that also writes the training records (generic-Z kernel).
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s);
 

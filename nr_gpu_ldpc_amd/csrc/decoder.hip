@@ -16,6 +16,8 @@ int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds
 
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
+template <int KIND, int ZC>
+int launch_decode_char(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 
 // Lifting sizes with a dedicated kernel (all offsets compile-time): the hot configurations.
 constexpr int kBG1Special = 384, kBG2Special = 208, kQCSpecial = 96;
@@ -91,6 +93,24 @@ int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s) {
                                   : launch_decode_half<kBG2, 0>(a, blocks, threads, lds, s);
     return a.Z == kQCSpecial ? launch_decode_half<kWifi, kQCSpecial>(a, blocks, threads, lds, s)
                              : launch_decode_half<kWifi, 0>(a, blocks, threads, lds, s);
+}
+
+int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s) {
+    const int blocks = (a.n_cw + a.cpb - 1) / a.cpb;
+    const int threads = a.cpb * a.Z;
+    const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;
+    if (threads > kMaxBlock || lds > (size_t)max_lds_bytes() || blocks < 1 || !a.llr_i8) {
+        set_last_error("launch shape exceeds the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
+    if (kind == kBG1)
+        return a.Z == kBG1Special ? launch_decode_char<kBG1, kBG1Special>(a, blocks, threads, lds, s)
+                                  : launch_decode_char<kBG1, 0>(a, blocks, threads, lds, s);
+    if (kind == kBG2)
+        return a.Z == kBG2Special ? launch_decode_char<kBG2, kBG2Special>(a, blocks, threads, lds, s)
+                                  : launch_decode_char<kBG2, 0>(a, blocks, threads, lds, s);
+    return a.Z == kQCSpecial ? launch_decode_char<kWifi, kQCSpecial>(a, blocks, threads, lds, s)
+                             : launch_decode_char<kWifi, 0>(a, blocks, threads, lds, s);
 }
 
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s) {

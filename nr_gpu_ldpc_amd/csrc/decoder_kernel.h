@@ -178,10 +178,15 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // syndrome after the last iteration; `emit_only_ok` then writes outputs only for codewords
 // whose hard decision passes it.  Returns, per lane, whether its codeword converged (early
 // exit) or passed the final check.
+// CHAR: the reference's CHAR_PRC schedule (its default build) -- signed-char channel LLRs
+// (a.llr_i8), R and dt truncated to signed char with saturation (the oracle's rule for the
+// conversions C++ leaves undefined), the APP between iterations clamped to [-128, 127], and a
+// saturated APP (127 or -128) entering the check node without R_old; all values are integers,
+// so every single-precision sum is exact.
 // HALF: the reference's HALF_PRC schedule -- binary16 channel LLRs (a.llr_half), and R, dt and
 // the APP between iterations rounded to binary16 where the reference stores them, all
 // arithmetic in single precision.
-template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false>
+template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false, bool CHAR = false>
 __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                                              const MixedCw* __restrict__ mcw, const bool sel = true,
                                              const bool final_check = false,
@@ -190,6 +195,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && (TRAIN || ZC > 0)), "mixed tiles use the any-Z plain/learned path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
+    static_assert(!(CHAR && (LEARNED || TRAIN || MIXED || HALF)), "the signed-char schedule is plain min-sum");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -253,8 +259,10 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     // ---- load channel LLRs
     const float* __restrict__ L = MIXED ? a.llr + mcw[cwc].llr_off : a.llr + (size_t)cwc * a.n_vn;
     const uint16_t* __restrict__ Lh = HALF ? a.llr_half + (size_t)cwc * a.n_vn : nullptr;
+    const int8_t* __restrict__ Lc = CHAR ? a.llr_i8 + (size_t)cwc * a.n_vn : nullptr;
     auto llr_at = [&](int n) -> float {
         if constexpr (HALF) return half_bits_to_float(Lh[n]);
+        else if constexpr (CHAR) return (float)Lc[n];
         else return L[n];
     };
     if (!idle) {
@@ -397,8 +405,12 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                         rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
                     } else {
                         rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
+                        // R_old as stored: the truncated magnitude saturated to a signed char
+                        if constexpr (CHAR) rv = __builtin_amdgcn_fmed3f(rv, -128.0f, 127.0f);
                     }
-                    const float qv = x - rv;
+                    float qv = x - rv;
+                    // a saturated APP (127 or -128) enters as it is
+                    if constexpr (CHAR) qv = fabsf(x + 0.5f) >= 127.5f ? x : qv;
                     q[k] = qv;
                     ro[k] = rv;
                     const float aq = fabsf(qv);
@@ -426,6 +438,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 }
                 float n1, n2;
                 if constexpr (LEARNED) { n1 = m1; n2 = m2; }
+                else if constexpr (CHAR) { n1 = truncf(0.75f * m1); n2 = truncf(0.75f * m2); }
                 else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
                 const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
                 const bool row_on = !MIXED || i < m_lane;  // MIXED: row kept by this codeword
@@ -444,9 +457,11 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     } else {
                         ms = eq ? n2s : n1s;
                     }
-                    const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
+                    float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
+                    if constexpr (CHAR) rn = __builtin_amdgcn_fmed3f(rn, -128.0f, 127.0f);
                     if constexpr (MIXED) dtv[k] = row_on ? rn - ro[k] : 0.0f;
                     else if constexpr (HALF) dtv[k] = round_half(rn - ro[k]);
+                    else if constexpr (CHAR) dtv[k] = __builtin_amdgcn_fmed3f(rn - ro[k], -128.0f, 127.0f);
                     else dtv[k] = rn - ro[k];
                 });
                 if constexpr (HALF) {  // R is stored in binary16; dt used the unrounded R_new
@@ -522,15 +537,29 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             for (int j = 0; j < NND; j++) {
                 float v = lds_ld(rAcc + j * Z4);
                 if constexpr (HALF) {  // the APP between iterations is stored in binary16
-                    v = round_half(v);
-                    lds_st(rAcc + j * Z4, v);
+                    if (!last) {
+                        v = round_half(v);
+                        lds_st(rAcc + j * Z4, v);
+                    }
+                } else if constexpr (CHAR) {  // ... or as a signed char
+                    if (!last) {
+                        v = __builtin_amdgcn_fmed3f(v, -128.0f, 127.0f);
+                        lds_st(rAcc + j * Z4, v);
+                    }
                 }
                 lds_st(rRd + j * 2 * Z4, v);
                 lds_st(rRd + j * 2 * Z4 + Z4, v);
             }
             if constexpr (HALF) {
+                if (!last) {
 #pragma unroll
-                for (int n = 0; n < ND1; n++) app1[n] = round_half(app1[n]);
+                    for (int n = 0; n < ND1; n++) app1[n] = round_half(app1[n]);
+                }
+            } else if constexpr (CHAR) {
+                if (!last) {
+#pragma unroll
+                    for (int n = 0; n < ND1; n++) app1[n] = __builtin_amdgcn_fmed3f(app1[n], -128.0f, 127.0f);
+                }
             }
         }
         const bool check = early || fin;
@@ -603,9 +632,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     return emitted || final_ok;
 }
 
-template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false>
+template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false, bool CHAR = false>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
-    decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF>(a, blockIdx.x, nullptr);
+    decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF, CHAR>(a, blockIdx.x, nullptr);
 }
 
 }  // namespace nrldpc

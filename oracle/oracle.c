@@ -249,12 +249,28 @@ float oracle_half_to_float(uint16_t h) {
 
 static inline float rh(float x) { return oracle_half_to_float(oracle_float_to_half(x)); }
 
-/* half_store: the HALF_PRC build, where R, dt and the APP between iterations are stored as
- * binary16 (src/ldpc_kernels.cu:158-160,266,304-306,351,385-386) and all arithmetic is single
- * precision; the last iteration's APP is not stored (:450-467). */
+/* float -> signed char of the CHAR_PRC build: truncation toward zero; values outside
+ * [-128, 127] (undefined behaviour in C++) saturate, and *oor counts them. */
+static inline float to_char(float x, long long *oor) {
+    float t = truncf(x);
+    if (t > 127.0f) { t = 127.0f; (*oor)++; }
+    else if (t < -128.0f) { t = -128.0f; (*oor)++; }
+    return t;
+}
+
+enum { PREC_FLOAT = 0, PREC_HALF = 1, PREC_CHAR = 2 };
+
+/* prec: PREC_HALF is the HALF_PRC build, where R, dt and the APP between iterations are stored
+ * as binary16 (src/ldpc_kernels.cu:158-160,266,304-306,351,385-386) and all arithmetic is single
+ * precision; the last iteration's APP is not stored (:450-467).  PREC_CHAR is the CHAR_PRC build
+ * (the reference's default, src/globals.h:12-13): LLRs, R, dt and the APP between iterations are
+ * signed chars, R and dt converted with to_char (:173-175, :304-306), the APP clamped to
+ * [-128, 127] (:385-390), and a saturated APP (127 or -128) enters the check node unchanged
+ * (:258-264); *oor counts the out-of-range conversions. */
 static void decode_one(const oracle_code *c, const edge_list *el, const float *llr, int n_iter,
                        int early_exit, const float *alpha, const float *beta, float *R, float *dt,
-                       float *app, uint8_t *hd, int *iters_out, int half_store) {
+                       float *app, uint8_t *hd, int *iters_out, int prec, long long *oor) {
+    const int half_store = prec == PREC_HALF, char_store = prec == PREC_CHAR;
     const int Z = c->Z, nvn = c->cols * Z;
     memcpy(app, llr, sizeof(float) * nvn);
     memset(R, 0, sizeof(float) * el->n_edges * Z);
@@ -273,7 +289,8 @@ static void decode_one(const oracle_code *c, const edge_list *el, const float *l
                     if (s >= Z) s -= Z;
                     const int col = el->ecol[e] * Z + s;
                     /* iteration 0 reads the channel LLR (:138); later ones APP - R_old (:266-267) */
-                    const float Q = (it == 0) ? app[col] : app[col] - R[e * Z + r];
+                    float Q = (it == 0) ? app[col] : app[col] - R[e * Z + r];
+                    if (char_store && (app[col] == 127.0f || app[col] == -128.0f)) Q = app[col];
                     const float Q_abs = fabsf(Q);
                     const int sq = Q < 0;
                     sign = sign * (float)(1 - sq * 2);
@@ -297,6 +314,12 @@ static void decode_one(const oracle_code *c, const edge_list *el, const float *l
                         Rn = sign * (float)sq * (alpha[b] * off);
                     }
                     const float Ro = (it == 0) ? 0.0f : R[e * Z + r];
+                    if (char_store) {
+                        const float Rc = to_char(Rn, oor);
+                        dt[e * Z + r] = (it == 0) ? Rc : to_char(Rc - Ro, oor); /* :173-174, :304 */
+                        R[e * Z + r] = Rc;
+                        continue;
+                    }
                     const float d = (it == 0) ? Rn : Rn - Ro; /* :177, :308-310 */
                     dt[e * Z + r] = half_store ? rh(d) : d;
                     R[e * Z + r] = half_store ? rh(Rn) : Rn;
@@ -314,6 +337,7 @@ static void decode_one(const oracle_code *c, const edge_list *el, const float *l
                     if (sf < 0) sf += Z;
                     a += dt[e * Z + sf];
                 }
+                if (char_store && it + 1 < n_iter) a = a > 127.0f ? 127.0f : (a < -128.0f ? -128.0f : a);
                 app[j * Z + s] = (half_store && it + 1 < n_iter) ? rh(a) : a;
             }
         }
@@ -333,7 +357,8 @@ typedef struct {
     uint8_t *hd;
     float *app;
     int *iters;
-    int half_store;
+    int prec;
+    long long oor;  /* out-of-range char conversions (PREC_CHAR) */
 } dec_job;
 
 static void *dec_worker(void *arg) {
@@ -348,16 +373,16 @@ static void *dec_worker(void *arg) {
     for (int w = b; w < e; w++) {
         decode_one(jb->c, jb->el, jb->llr + (size_t)w * nvn, jb->n_iter, jb->early_exit, jb->alpha,
                    jb->beta, R, dt, app, jb->hd + (size_t)w * nvn, jb->iters ? jb->iters + w : NULL,
-                   jb->half_store);
+                   jb->prec, &jb->oor);
         if (jb->app) memcpy(jb->app + (size_t)w * nvn, app, sizeof(float) * nvn);
     }
     free(R); free(dt); free(app);
     return NULL;
 }
 
-static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
-                        const float *alpha, const float *beta, uint8_t *hd, float *app, int *iters,
-                        int threads, int half_store) {
+static long long decode_many(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
+                              const float *alpha, const float *beta, uint8_t *hd, float *app, int *iters,
+                              int threads, int prec) {
     edge_list el;
     edges_build(c, &el);
     if (threads < 1) threads = 1;
@@ -366,7 +391,7 @@ static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_
     dec_job jobs[256];
     if (threads > 256) threads = 256;
     for (int t = 0; t < threads; t++) {
-        dec_job j = {c, &el, llr, n_cw, n_iter, early_exit, t, threads, alpha, beta, hd, app, iters, half_store};
+        dec_job j = {c, &el, llr, n_cw, n_iter, early_exit, t, threads, alpha, beta, hd, app, iters, prec, 0};
         jobs[t] = j;
         if (threads == 1) dec_worker(&jobs[t]);
         else pthread_create(&th[t], NULL, dec_worker, &jobs[t]);
@@ -374,11 +399,14 @@ static void decode_many(const oracle_code *c, const float *llr, int n_cw, int n_
     if (threads > 1)
         for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
     edges_free(&el);
+    long long oor = 0;
+    for (int t = 0; t < threads; t++) oor += jobs[t].oor;
+    return oor;
 }
 
 void oracle_decode(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
                    uint8_t *hd, float *app, int *iters, int threads) {
-    decode_many(c, llr, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, 0);
+    decode_many(c, llr, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, PREC_FLOAT);
 }
 
 void oracle_decode_half(const oracle_code *c, const uint16_t *llr, int n_cw, int n_iter, int early_exit,
@@ -386,8 +414,18 @@ void oracle_decode_half(const oracle_code *c, const uint16_t *llr, int n_cw, int
     const size_t n = (size_t)n_cw * c->cols * c->Z;
     float *f = malloc(sizeof(float) * (n ? n : 1));
     for (size_t i = 0; i < n; i++) f[i] = oracle_half_to_float(llr[i]);
-    decode_many(c, f, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, 1);
+    decode_many(c, f, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, PREC_HALF);
     free(f);
+}
+
+long long oracle_decode_char(const oracle_code *c, const int8_t *llr, int n_cw, int n_iter, int early_exit,
+                             uint8_t *hd, float *app, int *iters, int threads) {
+    const size_t n = (size_t)n_cw * c->cols * c->Z;
+    float *f = malloc(sizeof(float) * (n ? n : 1));
+    for (size_t i = 0; i < n; i++) f[i] = (float)llr[i];
+    const long long oor = decode_many(c, f, n_cw, n_iter, early_exit, NULL, NULL, hd, app, iters, threads, PREC_CHAR);
+    free(f);
+    return oor;
 }
 
 void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter,

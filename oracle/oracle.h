@@ -52,6 +52,12 @@ void oracle_decode(const oracle_code *c, const float *llr, int n_cw, int n_iter,
  * binary16, arithmetic in single precision (src/ldpc_kernels.cu, HALF_PRC branches). */
 void oracle_decode_half(const oracle_code *c, const uint16_t *llr, int n_cw, int n_iter, int early_exit,
                         uint8_t *hd, float *app, int *iters, int threads);
+/* The reference's CHAR_PRC build (its default): signed-char LLRs (quantize(0.5, x, 8)), R, dt and
+ * the APP between iterations, arithmetic in single precision; float -> char conversions truncate
+ * toward zero and saturate out of range (undefined in C++).  Returns the number of out-of-range
+ * conversions, the cases where the reference's own result is not defined. */
+long long oracle_decode_char(const oracle_code *c, const int8_t *llr, int n_cw, int n_iter, int early_exit,
+                             uint8_t *hd, float *app, int *iters, int threads);
 uint16_t oracle_float_to_half(float f);  /* round to nearest even */
 float oracle_half_to_float(uint16_t h);
 

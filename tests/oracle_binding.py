@@ -31,6 +31,9 @@ class Oracle:
         lib.oracle_decode_noms.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, P, P, P, P]
         lib.oracle_decode_half.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            P, P, P, ctypes.c_int]
+        lib.oracle_decode_char.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           P, P, P, ctypes.c_int]
+        lib.oracle_decode_char.restype = ctypes.c_longlong
         lib.oracle_float_to_half.argtypes = [ctypes.c_float]
         lib.oracle_float_to_half.restype = ctypes.c_uint16
         lib.oracle_half_to_float.argtypes = [ctypes.c_uint16]
@@ -82,6 +85,18 @@ class Oracle:
         self.lib.oracle_decode_half(ctypes.byref(c), llr_half.ctypes.data, n_cw, n_iter, int(early_exit),
                                     hd.ctypes.data, app.ctypes.data, iters.ctypes.data, threads)
         return hd, app, iters
+
+    def decode_char(self, c, llr_i8, n_iter, early_exit=False, threads=8):
+        """llr_i8: int8 LLRs [n_cw][cols*Z] (CHAR_PRC).  Returns hd, app, iters and the number of
+        out-of-range float -> char conversions (saturated)."""
+        llr_i8 = np.ascontiguousarray(llr_i8, dtype=np.int8)
+        n_cw, nvn = llr_i8.shape
+        hd = np.empty((n_cw, nvn), dtype=np.uint8)
+        app = np.empty((n_cw, nvn), dtype=np.float32)
+        iters = np.empty(n_cw, dtype=np.int32)
+        oor = self.lib.oracle_decode_char(ctypes.byref(c), llr_i8.ctypes.data, n_cw, n_iter, int(early_exit),
+                                          hd.ctypes.data, app.ctypes.data, iters.ctypes.data, threads)
+        return hd, app, iters, int(oor)
 
     def decode_noms(self, c, llr, n_iter, alpha, beta):
         llr = np.ascontiguousarray(llr, dtype=np.float32)

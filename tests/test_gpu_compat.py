@@ -74,7 +74,7 @@ def test_stream_packing_for_odd_lifting_size(nr, torch_gpu):
 
 
 def test_c_entry_point_matches_c_abi(tmp_path):
-    """ldpc_gpu_decoder(char*, char*) on int8 LLRs equals the C ABI decode of 0.5 * q."""
+    """ldpc_gpu_decoder(char*, char*) on int8 LLRs equals the C ABI's CHAR_PRC decode of them."""
     print("case C entry point BG1 Z=384", flush=True)
     exe = str(tmp_path / "c_entry")
     lib = os.path.join(ROOT, "nr_gpu_ldpc_amd", "lib")

@@ -59,11 +59,21 @@ def _dense_H(oracle, code):
     return H
 
 
-def dense_minsum(H, llr, n_iter, half=False):
+def dense_minsum(H, llr, n_iter, half=False, char=False):
     """Flooding scaled min-sum on a dense H, written from the reference kernels' arithmetic.
-    half: the HALF_PRC build (R, dt and the APP between iterations rounded to binary16)."""
+    half: the HALF_PRC build (R, dt and the APP between iterations rounded to binary16).
+    char: the CHAR_PRC build (R and dt truncated to signed char, saturating; the APP between
+    iterations clamped to [-128, 127]; a saturated APP enters the check node as it is).
+    Returns hard decisions, APP and, for char, the number of out-of-range conversions."""
     M, N = H.shape
     st = (lambda v: np.asarray(v, F).astype(np.float16).astype(F)) if half else (lambda v: np.asarray(v, F))
+    oor = [0]
+
+    def to_char(v):
+        t = np.trunc(np.asarray(v, F))
+        oor[0] += int(np.count_nonzero((t > 127) | (t < -128)))
+        return np.clip(t, -128, 127).astype(F)
+
     app = llr.astype(F).copy()
     R = np.zeros((M, N), dtype=F)
     nbr = [np.nonzero(H[m])[0] for m in range(M)]
@@ -72,6 +82,8 @@ def dense_minsum(H, llr, n_iter, half=False):
         for m in range(M):
             cols = nbr[m]
             Q = app[cols] if t == 0 else (app[cols] - R[m, cols]).astype(F)
+            if char:
+                Q = np.where((app[cols] == 127) | (app[cols] == -128), app[cols], Q).astype(F)
             absq = np.abs(Q)
             vals = np.concatenate([absq, [F(1000.0), F(1000.0)]]).astype(F)
             idx = int(np.argmin(absq)) if absq.min() < F(1000.0) else 0  # first strict minimum
@@ -81,13 +93,20 @@ def dense_minsum(H, llr, n_iter, half=False):
             sign = F(np.prod(sq))
             mag = np.where(np.arange(len(cols)) == idx, m2, m1).astype(F)
             Rn = ((F(0.75) * sign) * sq * mag).astype(F)
+            if char:
+                Rc = to_char(Rn)
+                dt[m, cols] = Rc if t == 0 else to_char(Rc - R[m, cols])
+                R[m, cols] = Rc
+                continue
             dt[m, cols] = st(Rn if t == 0 else (Rn - R[m, cols]).astype(F))
             R[m, cols] = st(Rn)
         for m in range(M):  # ascending block row per column
             cols = nbr[m]
             app[cols] = (app[cols] + dt[m, cols]).astype(F)
         if t + 1 < n_iter:
-            app = st(app)
+            app = np.clip(app, -128, 127).astype(F) if char else st(app)
+    if char:
+        return (app <= 0).astype(np.uint8), app, oor[0]
     return (app <= 0).astype(np.uint8), app
 
 
@@ -264,3 +283,59 @@ def test_oracle_half_matches_dense_restatement(oracle, kind, Z, m):
     # and it differs from the single-precision decoder somewhere (the storage points matter)
     f_hd, f_app, _ = oracle.decode(code, llr.astype(F), 5)
     assert not np.array_equal(f_app, app)
+
+
+def _quantize(x, D=0.5, B=8):
+    """The reference's quantize(D, x, B) (src/host_helpers.cpp:46-58): floor(x / D), clamped to
+    [-2^(B-1), 2^(B-1) - 1]."""
+    q = np.floor(np.asarray(x, np.float64) / D)
+    return np.clip(q, -(1 << (B - 1)), (1 << (B - 1)) - 1).astype(np.int8)
+
+
+@pytest.mark.parametrize("kind,Z,m,scale", [(OR_WIFI, 5, 0, 20.0), (OR_BG2, 2, 0, 40.0), (OR_BG1, 3, 0, 30.0),
+                                            (OR_BG2, 3, 6, 90.0)])
+def test_oracle_char_matches_dense_restatement(oracle, kind, Z, m, scale):
+    """CHAR_PRC: the oracle equals the dense restatement with signed-char storage points, on
+    LLRs large enough that saturated APPs and out-of-range conversions occur."""
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    rng = np.random.default_rng(Z * 17 + kind)
+    n = H.shape[1]
+    llr = _quantize(rng.standard_normal((3, n)) * scale + 0.3 * scale)
+    if kind != OR_WIFI:
+        llr[:, :2 * Z] = 0
+    hd, app, _, oor = oracle.decode_char(code, llr, 6)
+    total = 0
+    for w in range(3):
+        dhd, dapp, door = dense_minsum(H, llr[w].astype(F), 6, char=True)
+        assert np.array_equal(hd[w], dhd)
+        assert np.array_equal(app[w], dapp)
+        total += door
+    assert oor == total
+    assert (np.abs(app) >= 127).any()  # saturation is exercised
+    if (kind, Z) == (OR_BG2, 2):
+        assert oor > 0  # and out-of-range conversions
+
+
+def test_quantize_matches_reference_rule():
+    x = np.array([-100.0, -64.2, -0.4, -0.0, 0.0, 0.2, 0.5, 0.74, 63.4, 63.5, 70.0])
+    assert _quantize(x).tolist() == [-128, -128, -1, 0, 0, 0, 1, 1, 126, 127, 127]
+
+
+def test_char_workload_has_no_out_of_range_conversions(oracle, nr):
+    """On the benchmark workload (BG1 Z=384, Eb/N0 1.0-2.0 dB, T=10, LLRs from quantize(0.5, x,
+    8)) no float -> char conversion of the CHAR_PRC schedule leaves [-128, 127], so its results
+    there are fully defined by C++ (truncation toward zero) and the saturating rule never acts."""
+    code = nr.Code(1, 384, 0, -1)  # host-only context: encoder only
+    rng = np.random.default_rng(8)
+    info = rng.integers(0, 2, (3, code.K), dtype=np.uint8)
+    cw = code.encode_host(info)
+    oc = oracle.code(OR_BG1, 384)
+    for ebn0 in (1.0, 1.5, 2.0):
+        sigma = nr.noise_stddev(ebn0, 1 / 3)
+        x = 1.0 - 2.0 * cw[:, 2 * 384:].astype(F)
+        y = (x + sigma * rng.standard_normal(x.shape).astype(F)).astype(F)
+        llr = np.zeros((3, code.n_vn), F)
+        llr[:, 2 * 384:] = (y * F(2.0)) / F(sigma * sigma)
+        _, _, _, oor = oracle.decode_char(oc, _quantize(llr), 10)
+        assert oor == 0, ebn0
