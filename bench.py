@@ -12,6 +12,8 @@ Also measured, each in its own object of the same JSON line:
     1..8192, each batch an LLR upload, a replayed HIP graph (plan upload, decode) and a
     hard-decision download, plain min-sum against learned weights with plain fallback; batch
     by batch (latency) and pipelined over two streams (throughput);
+  * ``transport_block``: the TS 38.212 receive chain (de-rate-matching, decode, CRC checks) for
+    one 999,576-bit transport block (119 code blocks);
   * ``stream_overlap``: the same traffic as batches of 16 / 256 / 2048 codewords over 1, 2 and 4
     streams round-robin (host planning and PCIe copies overlapped with decoding);
   * ``cpu_baseline``: the C oracle (same schedule) on this host's cores.
@@ -529,6 +531,51 @@ def bench_pipeline(torch, nr, args):
     return out
 
 
+def bench_transport_block(torch, nr, args):
+    """The TS 38.212 receive chain around the decoder for one large transport block: de-rate-
+    matching of G channel LLRs into C code blocks, the fused decode (T=10, early exit), the
+    code-block CRC checks and the transport-block assembly, all on the device."""
+    A, rate, G, Qm = 999_576, 1.0 / 3.0, 3_000_000, 4  # a TS 38.214 size: C = 119, K' = 8424
+    t = nr.TransportBlock(A, rate, G, Qm=Qm)
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2, A, dtype=np.uint8)
+    f, _ = t.encode_host(a)
+    sigma = np.float32(0.7)
+    y = (1.0 - 2.0 * f.astype(np.float32)) + sigma * rng.standard_normal(G).astype(np.float32)
+    llr = torch.from_numpy(((y * np.float32(2)) / (sigma * sigma)).astype(np.float32)).cuda()
+    code = nr.Code(t.bg, t.Z)
+    d_cb = torch.empty((t.C, t.n_vn), dtype=torch.float32, device="cuda")
+    d_hd = torch.empty((t.C, code.hd_bytes), dtype=torch.uint8, device="cuda")
+    ok = torch.zeros(t.C, dtype=torch.int32, device="cuda")
+    bits = torch.zeros(t.B, dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+
+    def chain():
+        t.derate(llr.data_ptr(), d_cb.data_ptr(), stream=s)
+        code.decode(d_cb.data_ptr(), t.C, N_ITER, d_hd.data_ptr(), flags=nr.EARLY_EXIT, stream=s)
+        t.check(d_hd.data_ptr(), ok.data_ptr(), bits.data_ptr(), stream=s)
+
+    chain()
+    torch.cuda.synchronize()
+    reps = max(5, args.steps)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for k in range(reps):
+        chain()
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    per = [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
+    passed = int(ok.sum().item())
+    ok_tb = passed == t.C and bool(torch.equal(bits[:A].cpu(), torch.from_numpy(a)))
+    return {"workload": f"one transport block: A={A} bits, rate 1/3, G={G} bits at Qm={Qm}, BPSK-equivalent "
+                        f"LLRs at sigma {float(sigma)}; C={t.C} BG{t.bg} Z={t.Z} code blocks, T<=10 with early exit",
+            "info_gbps": round(A / (statistics.median(per) * 1e6), 3),
+            "latency_ms": {"p50": round(statistics.median(per), 3), "p99": round(pct(per, 0.99), 3),
+                           "kind": "device time: de-rate-matching + decode + CRC checks + assembly"},
+            "code_blocks": t.C, "code_blocks_passed": passed, "payload_recovered": ok_tb}
+
+
 def bench_operating_point(torch, nr, args):
     """Info throughput at the FER 1e-3 operating point of each decoder (BG1 Z=384, T <= 10 with
     early exit, batches of 4096): plain min-sum at its FER 1e-3 Eb/N0, the learned decoder (with
@@ -652,6 +699,7 @@ def main():
         out["fer1e-3_operating_point"] = op
     if not args.skip_mixed:
         out["mixed_batch"] = bench_mixed(torch, nr, args)
+        out["transport_block"] = bench_transport_block(torch, nr, args)
     if not args.skip_stream:
         out["combined_stream"] = bench_stream(torch, nr, args)
     if not args.skip_stream:
