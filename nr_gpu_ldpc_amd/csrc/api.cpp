@@ -243,6 +243,7 @@ int ldpc_syndrome_async(ldpc_ctx* x, void* stream, const uint8_t* hd, int n_cw, 
     a.edge_vm = x->d_edge_vm;
     a.weight = weight;
     a.n_edges = x->code.nnz();
+    a.cols = x->code.cols;
     return syndrome_launch(a, n_cw, static_cast<hipStream_t>(stream));
 }
 

@@ -29,6 +29,7 @@ struct SyndromeArgs {
     const int* edge_vm;
     int* weight;
     int n_edges;           // row_start[rows]
+    int cols;              // block columns in use
 };
 
 struct ErrorArgs {

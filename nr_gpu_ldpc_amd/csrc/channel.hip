@@ -172,43 +172,87 @@ __global__ void __launch_bounds__(256) k_channel(ChannelArgs a) {
     }
 }
 
-// Syndrome weight per codeword from packed hard decisions.  The codeword's packed decisions
-// and the code's edge list are staged in LDS (coalesced loads), so every parity check gathers
-// its bits from LDS.  Edge e is kept as (block column * Z) | (V mod Z) << 16.
+// Syndrome weight per codeword from packed hard decisions.  The codeword's decisions and the
+// code's edge list (block column | V mod Z << 16, row-major) are staged in LDS.
+//   Z >= 32: 32 parity checks per operation.  Block column j is laid out twice over, as the
+//   bit string D_j[b] = HD[j*Z + (b mod Z)], b < 2Z; checks 32w..32w+31 of block row i are
+//   then the XOR over the row's edges of the 32 bits of D_j starting at bit 32w + V (one
+//   funnel shift each), and the weight is the popcount of the valid bits.
+//   Z < 32: one check at a time.
 __global__ void __launch_bounds__(256) k_syndrome(SyndromeArgs a) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char sm[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t smw[];
     const int cw = blockIdx.x;
-    const int Z = a.Z, rows = a.rows;
-    int* rs = reinterpret_cast<int*>(sm);                 // [rows + 1]
-    uint32_t* et = reinterpret_cast<uint32_t*>(rs + rows + 1);
-    const int n_e = a.row_start[rows];
-    uint8_t* hb = reinterpret_cast<uint8_t*>(et + n_e);    // [hd_bytes]
+    const int Z = a.Z, rows = a.rows, cols = a.cols;
+    const int n_e = a.n_edges;
+    int* rs = reinterpret_cast<int*>(smw);       // [rows + 1]
+    uint32_t* et = smw + (rows + 1);              // [n_e]
+    uint32_t* W = et + n_e;                       // stream words, zero padded: [nw + 2]
+    const int nw = (a.hd_bytes + 3) / 4;
+    const int WD = (2 * Z + 31) / 32 + 2;         // words of one doubled block column
+    uint32_t* D = W + nw + 2;                     // [cols][WD] (Z >= 32)
     for (int i = threadIdx.x; i <= rows; i += blockDim.x) rs[i] = a.row_start[i];
     for (int e = threadIdx.x; e < n_e; e += blockDim.x)
-        et[e] = (uint32_t)(a.edge_col[e] * Z) | ((uint32_t)a.edge_vm[e] << 16);
+        et[e] = (uint32_t)a.edge_col[e] | ((uint32_t)a.edge_vm[e] << 16);
     const uint8_t* hd = a.hd + (size_t)cw * a.hd_bytes;
-    for (int b = threadIdx.x; b < a.hd_bytes; b += blockDim.x) hb[b] = hd[b];
-    __syncthreads();
-    int w = 0;
-    for (int chk = threadIdx.x; chk < rows * Z; chk += blockDim.x) {
-        const int i = chk / Z, r = chk - i * Z;
-        uint32_t p = 0;
-        const int e1 = rs[i + 1];
-        for (int e = rs[i]; e < e1; e++) {
-            const uint32_t t = et[e];
-            int c = r + (int)(t >> 16);
-            if (c >= Z) c -= Z;
-            const int n = (int)(t & 0xFFFFu) + c;
-            p ^= (uint32_t)(hb[n >> 3] >> (n & 7));
+    for (int k = threadIdx.x; k < nw + 2; k += blockDim.x) {
+        uint32_t v = 0;
+        for (int q = 0; q < 4; q++) {
+            const int b = 4 * k + q;
+            if (b < a.hd_bytes) v |= (uint32_t)hd[b] << (8 * q);
         }
-        w += (int)(p & 1u);
+        W[k] = v;
     }
     __syncthreads();
-    int* red = reinterpret_cast<int*>(sm);
-    red[threadIdx.x] = w;
+    auto ext = [](const uint32_t* w, int p) {  // 32 bits of w starting at bit p
+        return __builtin_amdgcn_alignbit(w[(p >> 5) + 1], w[p >> 5], (uint32_t)(p & 31));
+    };
+    int wsum = 0;
+    if (Z >= 32) {
+        for (int idx = threadIdx.x; idx < cols * WD; idx += blockDim.x) {
+            const int j = idx / WD, w = idx - j * WD;
+            const int b0 = 32 * w;
+            uint32_t v = 0;
+            if (b0 < 2 * Z) {
+                const int p = b0 >= Z ? b0 - Z : b0;
+                const int n1 = Z - p;  // bits before the column wraps
+                v = ext(W, j * Z + p);
+                if (n1 < 32) v = (v & ((1u << n1) - 1u)) | (ext(W, j * Z) << n1);
+            }
+            D[idx] = v;
+        }
+        __syncthreads();
+        const int nwr = (Z + 31) / 32;
+        for (int item = threadIdx.x; item < rows * nwr; item += blockDim.x) {
+            const int i = item / nwr, w = item - i * nwr;
+            uint32_t par = 0;
+            for (int e = rs[i]; e < rs[i + 1]; e++) {
+                const uint32_t t = et[e];
+                par ^= ext(D + (int)(t & 0xFFFFu) * WD, 32 * w + (int)(t >> 16));
+            }
+            const int valid = Z - 32 * w;
+            if (valid < 32) par &= (1u << valid) - 1u;
+            wsum += __popc(par);
+        }
+    } else {
+        for (int chk = threadIdx.x; chk < rows * Z; chk += blockDim.x) {
+            const int i = chk / Z, r = chk - i * Z;
+            uint32_t p = 0;
+            for (int e = rs[i]; e < rs[i + 1]; e++) {
+                const uint32_t t = et[e];
+                int c = r + (int)(t >> 16);
+                if (c >= Z) c -= Z;
+                const int n = (int)(t & 0xFFFFu) * Z + c;
+                p ^= W[n >> 5] >> (n & 31);
+            }
+            wsum += (int)(p & 1u);
+        }
+    }
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    int* red = reinterpret_cast<int*>(smw);
+    red[threadIdx.x] = wsum;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
     if (threadIdx.x == 0) a.weight[cw] = red[0];
@@ -325,9 +369,10 @@ int channel_launch(const ChannelArgs& a, int n_cw, hipStream_t s) {
 }
 
 int syndrome_launch(const SyndromeArgs& a, int n_cw, hipStream_t s) {
-    const size_t lds = std::max<size_t>(256 * sizeof(int),
-                                        (size_t)(a.rows + 1) * sizeof(int) + (size_t)a.n_edges * sizeof(uint32_t) +
-                                            (size_t)a.hd_bytes);
+    const size_t nw = (size_t)(a.hd_bytes + 3) / 4 + 2;
+    const size_t wd = (size_t)(2 * a.Z + 31) / 32 + 2;
+    const size_t words = (size_t)(a.rows + 1) + (size_t)a.n_edges + nw + (a.Z >= 32 ? (size_t)a.cols * wd : 0);
+    const size_t lds = std::max<size_t>(256 * sizeof(int), words * sizeof(uint32_t));
     hipLaunchKernelGGL(k_syndrome, dim3(n_cw), dim3(256), lds, s, a);
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;

@@ -2,7 +2,9 @@
 import numpy as np
 import pytest
 
-from tests.oracle_binding import OR_BG2
+from tests.oracle_binding import OR_BG1, OR_BG2, OR_WIFI
+
+KIND = {1: OR_BG1, 2: OR_BG2, 3: OR_WIFI}
 
 pytestmark = pytest.mark.gpu
 F = np.float32
@@ -35,24 +37,28 @@ def test_channel_bit_exact(nr, oracle, torch_gpu, kind, Z, m):
         assert np.array_equal(llr[w].view(np.uint32), want.view(np.uint32))
 
 
-def test_syndrome_kernel_matches_oracle(nr, oracle, torch_gpu):
-    print("case syndrome BG2 Z=52", flush=True)
+# Z >= 32 runs 32 checks per word operation, Z < 32 one check at a time
+@pytest.mark.parametrize("kind,Z,m", [(2, 52, 0), (1, 384, 0), (1, 36, 0), (3, 96, 0), (2, 104, 12), (2, 13, 0),
+                                      (1, 2, 0)])
+def test_syndrome_kernel_matches_oracle(nr, oracle, torch_gpu, kind, Z, m):
+    print(f"case syndrome kind={kind} Z={Z} m={m}", flush=True)
     torch = torch_gpu
-    code = nr.Code(2, 52)
-    rng = np.random.default_rng(0)
-    host = nr.Code(2, 52, 0, -1)
+    code = nr.Code(kind, Z, m)
+    rng = np.random.default_rng(Z + kind)
+    host = nr.Code(kind, Z, m, -1)
     cw = host.encode_host(rng.integers(0, 2, (6, code.K), dtype=np.uint8))
     cw[1, 5] ^= 1
     cw[2, rng.integers(0, code.n_vn, 40)] ^= 1
     cw[3] = rng.integers(0, 2, code.n_vn)
+    cw[4, -1] ^= 1
     packed = np.packbits(cw, axis=1, bitorder="little")
     d = torch.from_numpy(packed).cuda()
     w = torch.zeros(6, dtype=torch.int32, device="cuda")
     code.syndrome(d.data_ptr(), 6, w.data_ptr())
     torch.cuda.synchronize()
-    oc = oracle.code(OR_BG2, 52)
+    oc = oracle.code(KIND[kind], Z, m)
     assert w.cpu().tolist() == [oracle.syndrome(oc, cw[i]) for i in range(6)]
-    assert w[0] == 0 and w[1] > 0
+    assert w[0] == 0 and w[1] > 0 and w[4] > 0
 
 
 def test_count_errors(nr, torch_gpu):
