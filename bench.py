@@ -645,15 +645,17 @@ def bench_reference_example():
         return None
     args = ["--code", "qc", "--Z", "96", "--iters", "5", "--snr-min", "0", "--snr-max", "10", "--snr-step", "1",
             "--codewords", "80", "--cw-per-mcw", "8", "--mcw", "10", "--streams", "8", "--seed", "69012"]
-    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
-    if r.returncode != 0:
-        return {"error": r.stderr[-500:]}
-    lines = [json.loads(x) for x in r.stdout.splitlines()]
-    return {"workload": "12 x 24 QC code, Z=96 (K=1152, N=2304), flooding min-sum T=5, 640 frames per SNR point "
-                        "(80 unique x 8 streams), host rand/drand48 data as the reference generates it",
-            "command": "nr_gpu_ldpc_amd/bin/ldpc_ber " + " ".join(args),
-            "points": [{k: x[k] for k in ("snr_db", "ber", "fer", "best_latency_us", "user_throughput_mbps")}
-                       for x in lines]}
+    out = {"workload": "12 x 24 QC code, Z=96 (K=1152, N=2304), flooding min-sum T=5, 640 frames per SNR point "
+                       "(80 unique x 8 streams), host rand/drand48 data as the reference generates it",
+           "command": "nr_gpu_ldpc_amd/bin/ldpc_ber " + " ".join(args)}
+    # FP32 (the configuration's LLRs) and the reference's default CHAR_PRC build (--prec char)
+    for key, extra in (("points", []), ("char_build_points", ["--prec", "char"])):
+        r = subprocess.run([exe] + args + extra, capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            return {"error": r.stderr[-500:]}
+        out[key] = [{k: x[k] for k in ("snr_db", "ber", "fer", "best_latency_us", "user_throughput_mbps")}
+                    for x in (json.loads(line) for line in r.stdout.splitlines())]
+    return out
 
 
 def main():

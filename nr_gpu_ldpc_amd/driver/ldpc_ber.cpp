@@ -8,10 +8,13 @@
 //
 //   ldpc_ber [--code nr|qc] [--info-bits A] [--rate R] [--Z Z] [--iters T] [--snr-min S0]
 //            [--snr-max S1] [--snr-step dS] [--codewords N] [--cw-per-mcw C] [--mcw M]
-//            [--streams S] [--seed X] [--dump PREFIX]
+//            [--streams S] [--seed X] [--dump PREFIX] [--prec float|char]
 //
-// --dump PREFIX writes the first batch's LLRs (float32 [n_cw][N_before_puncture]) and packed
-// hard decisions of stream 0 to PREFIX.llr / PREFIX.hd, for checking against the oracle.
+// --prec char runs the reference's default CHAR_PRC build (LLRs quantised by quantize(0.5, x,
+// 8), the signed-char schedule); float (the default here) the FP32 one.
+// --dump PREFIX writes the first batch's LLRs (float32, or int8 with --prec char,
+// [n_cw][N_before_puncture]) and packed hard decisions of stream 0 to PREFIX.llr / PREFIX.hd,
+// for checking against the oracle.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -32,6 +35,7 @@ struct Options {
     int codewords = 5000, cw_per_mcw = 2, mcw = 20, streams = 6;
     long seed = 69012;
     std::string dump;
+    std::string prec = "float";
 };
 
 double now_ms() {
@@ -65,14 +69,15 @@ error_result measure_ber(const host_mem<T>* h, int n_streams, int n_cw_per_mcw, 
     return e;
 }
 
-gpu_result run_test_on_batch(int cws_in_iteration, float sigma, host_mem<float>* h, dev_mem<float>* d,
+template <typename T>
+gpu_result run_test_on_batch(int cws_in_iteration, float sigma, host_mem<T>* h, dev_mem<T>* d,
                              const cuda_grid* g, const ldpc_params* ldpc, const params* p, const Options& o,
                              bool first) {
-    generate_new_codewords<float>(p, h, ldpc, cws_in_iteration, sigma);
-    const float latency = test_latency<float>(d, h, g, ldpc->n_iterations);
-    const float run_time = test_throughput<float>(d, h, g, ldpc->n_iterations, p->n_streams);
-    const error_result e = measure_ber<float>(h, p->n_streams, p->n_cw_per_mcw, p->n_mcw, ldpc->N_before_puncture,
-                                              ldpc->K);
+    generate_new_codewords<T>(p, h, ldpc, cws_in_iteration, sigma);
+    const float latency = test_latency<T>(d, h, g, ldpc->n_iterations);
+    const float run_time = test_throughput<T>(d, h, g, ldpc->n_iterations, p->n_streams);
+    const error_result e = measure_ber<T>(h, p->n_streams, p->n_cw_per_mcw, p->n_mcw, ldpc->N_before_puncture,
+                                          ldpc->K);
     if (first && !o.dump.empty()) {
         FILE* f = fopen((o.dump + ".llr").c_str(), "wb");
         if (f) { fwrite(h->llr_cuda[0], 1, h->mem_size_llr_cuda, f); fclose(f); }
@@ -82,7 +87,8 @@ gpu_result run_test_on_batch(int cws_in_iteration, float sigma, host_mem<float>*
     return {latency, run_time, e.n_bits, e.bit_error, e.n_frames, e.frame_error};
 }
 
-void run_exp(const ldpc_params* ldpc, host_mem<float>* h, dev_mem<float>* d, const cuda_grid* g, const params* p,
+template <typename T>
+void run_exp(const ldpc_params* ldpc, host_mem<T>* h, dev_mem<T>* d, const cuda_grid* g, const params* p,
              const Options& o) {
     bool first = true;
     for (float snr = p->snr_min; snr <= p->snr_max + 1e-6f; snr += p->snr_step) {
@@ -124,9 +130,29 @@ bool parse(int argc, char** argv, Options* o) {
         else if (a == "--streams") o->streams = atoi(v);
         else if (a == "--seed") o->seed = atol(v);
         else if (a == "--dump") o->dump = v;
+        else if (a == "--prec") o->prec = v;
         else return false;
     }
-    return o->snr_step > 0 && o->cw_per_mcw > 0 && o->mcw > 0 && o->streams > 0 && o->iters > 0;
+    return o->snr_step > 0 && o->cw_per_mcw > 0 && o->mcw > 0 && o->streams > 0 && o->iters > 0 &&
+           (o->prec == "float" || o->prec == "char");
+}
+
+// T = float: the FP32 build; T = char: the reference's default CHAR_PRC build.
+template <typename T>
+void run(const ldpc_params& ldpc, params& p, const Options& o) {
+    host_mem<T> host = setup_host<T>(&p, ldpc.K, ldpc.N_before_puncture);
+    dev_mem<T> dev = setup_dev_memory<T>(&p, &ldpc, &host);
+    const cuda_grid grid = setup_gpu_grid(&ldpc, p.n_cw_per_mcw, p.n_mcw);
+    put_h_compact_and_params_in_constant_memory(&ldpc, &p, &grid);
+    // one untimed decode per stream first: the first launch of the process loads the code
+    // object and the first use of a stream sets it up, which would otherwise land in the first
+    // SNR point's latency and throughput
+    for (int s = 0; s < p.n_streams; s++) {
+        memset(host.llr_cuda[s], 0, host.mem_size_llr_cuda);
+        perform_gpu_ldpc_decoding<T>(&dev, &host, &grid, ldpc.n_iterations, s);
+    }
+    (void)hipDeviceSynchronize();
+    run_exp<T>(&ldpc, &host, &dev, &grid, &p, o);
 }
 
 }  // namespace
@@ -136,7 +162,7 @@ int main(int argc, char** argv) {
     if (!parse(argc, argv, &o)) {
         fprintf(stderr, "usage: ldpc_ber [--code nr|qc] [--info-bits A] [--rate R] [--Z Z] [--iters T] "
                         "[--snr-min S] [--snr-max S] [--snr-step S] [--codewords N] [--cw-per-mcw C] [--mcw M] "
-                        "[--streams S] [--seed X] [--dump PREFIX]\n");
+                        "[--streams S] [--seed X] [--dump PREFIX] [--prec float|char]\n");
         return 2;
     }
     params p = {o.snr_min, o.snr_max, o.snr_step, o.codewords, o.info_bits, o.rate, o.cw_per_mcw, o.mcw, o.streams};
@@ -145,18 +171,7 @@ int main(int argc, char** argv) {
     p.code_rate = ldpc.rate;
     fprintf(stderr, "code bg=%d Z=%d K=%d N=%d rows=%d cols=%d iters=%d streams=%d batch=%dx%d\n", ldpc.bg_index,
             ldpc.Z, ldpc.K, ldpc.N, ldpc.H_rows, ldpc.H_cols, ldpc.n_iterations, p.n_streams, p.n_mcw, p.n_cw_per_mcw);
-    host_mem<float> host = setup_host<float>(&p, ldpc.K, ldpc.N_before_puncture);
-    dev_mem<float> dev = setup_dev_memory<float>(&p, &ldpc, &host);
-    const cuda_grid grid = setup_gpu_grid(&ldpc, p.n_cw_per_mcw, p.n_mcw);
-    put_h_compact_and_params_in_constant_memory(&ldpc, &p, &grid);
-    // one untimed decode per stream first: the first launch of the process loads the code
-    // object and the first use of a stream sets it up, which would otherwise land in the first
-    // SNR point's latency and throughput
-    for (int s = 0; s < p.n_streams; s++) {
-        memset(host.llr_cuda[s], 0, host.mem_size_llr_cuda);
-        perform_gpu_ldpc_decoding<float>(&dev, &host, &grid, ldpc.n_iterations, s);
-    }
-    (void)hipDeviceSynchronize();
-    run_exp(&ldpc, &host, &dev, &grid, &p, o);
+    if (o.prec == "char") run<char>(ldpc, p, o);
+    else run<float>(ldpc, p, o);
     return 0;
 }

@@ -64,6 +64,9 @@ def test_cpp_host_api_exported(nr):
             "put_h_compact_and_params_in_constant_memory(ldpc_params const*, params const*, cuda_grid const*)",
             "void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, cuda_grid const*, int, int)",
             "nr_gpu_ldpc_context()", "ldpc_gpu_decoder",
+            "host_mem<char> setup_host<char>(params const*, unsigned int, unsigned int)",
+            "void perform_gpu_ldpc_decoding<char>(dev_mem<char>*, host_mem<char>*, cuda_grid const*, int, int)",
+            "void generate_new_codewords<char>(params const*, host_mem<char>*, ldpc_params const*, int, float)",
             "put_params_in_constant_mem(unsigned int, unsigned int, unsigned int, unsigned int, unsigned int, unsigned int)",
             "nr_gpu_ldpc_fill_compact(h_element*, int, int, h_element*, int, int, ldpc_params const*)",
             "nr_gpu_ldpc_load_compact(h_element const*, int, int, h_element const*, int, int, ldpc_params const*)"]

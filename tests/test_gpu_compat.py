@@ -15,12 +15,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "nr_gpu_ldpc_amd", "bin", "ldpc_ber")
 
 
-def _run(args, tmp_path):
+def _run(args, tmp_path, dtype=np.float32):
     prefix = str(tmp_path / "dump")
     r = subprocess.run([DRIVER] + args + ["--dump", prefix], capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     lines = [json.loads(x) for x in r.stdout.splitlines()]
-    llr = np.fromfile(prefix + ".llr", dtype=np.float32)
+    llr = np.fromfile(prefix + ".llr", dtype=dtype)
     hd = np.fromfile(prefix + ".hd", dtype=np.uint8)
     return lines, llr, hd
 
@@ -53,6 +53,23 @@ def test_driver_qc_z96_sweep(oracle, tmp_path):
     assert fer[0] > 0.5 and fer[-1] == 0.0
     assert all(a >= b - 0.05 for a, b in zip(fer, fer[1:]))
     _check_against_oracle(oracle, oracle.code(OR_WIFI, 96), llr, hd, 24 * 96, 5)
+
+
+def test_driver_char_build_qc_z96_sweep(oracle, tmp_path):
+    """The reference's default CHAR_PRC build through the same driver: int8 LLRs from
+    quantize(0.5, x, 8), decoded bit for bit as the oracle's CHAR_PRC restatement decodes them."""
+    print("case driver CHAR_PRC QC Z=96 SNR 0..10", flush=True)
+    args = ["--code", "qc", "--Z", "96", "--iters", "5", "--snr-min", "0", "--snr-max", "10", "--snr-step", "1",
+            "--codewords", "80", "--cw-per-mcw", "8", "--mcw", "10", "--streams", "8", "--prec", "char"]
+    lines, q, hd = _run(args, tmp_path, dtype=np.int8)
+    assert len(lines) == 11 and all(x["frames"] == 640 for x in lines)
+    fer = [x["fer"] for x in lines]
+    assert fer[0] > 0.5 and fer[-1] == 0.0
+    n_vn = 24 * 96
+    ohd, _, _, oor = oracle.decode_char(oracle.code(OR_WIFI, 96), q.reshape(-1, n_vn), 5)
+    want = np.packbits(ohd.reshape(-1), bitorder="little")
+    assert np.array_equal(hd[:want.size], want)
+    assert oor == 0
 
 
 def test_stream_packing_for_odd_lifting_size(nr, torch_gpu):
