@@ -271,7 +271,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             const float v = llr_at(j * Z + r) + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
             lds_st(rRd + j * 2 * Z4, v);
             lds_st(rRd + j * 2 * Z4 + Z4, v);
-            lds_st(rAcc + j * Z4, v);
+            if constexpr (CHAR) lds_stu(rAcc + j * Z4, (uint32_t)(int)v);  // integer accumulator
+            else lds_st(rAcc + j * Z4, v);
         }
     }
     if (!idle) static_for<NND, S::COLS>([&](auto JC) {
@@ -315,7 +316,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             }
 #pragma unroll 1
             for (int j = 0; j < NND; j++) {
-                const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                float x;
+                if constexpr (CHAR) x = (float)(int)lds_ldu(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                else x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
                 lds_stb(eb + j * Z + er, x > 0.f ? 0 : 1);
                 if (ap_out) ap_out[j * Z + er] = x;
             }
@@ -477,8 +480,26 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
                 else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
             }  // !idle
-            if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
+            // CHAR: every value is an integer, so the variable-node sums are exact in any order and
+            // need no ordering barriers: each edge adds its dt to an integer accumulator atomically.
+            if constexpr (S::T.epoch_start[i] != 0 && i != 0 && !CHAR) __syncthreads();
             if (idle) return;
+            if constexpr (CHAR) {
+                static_for<0, D>([&](auto KC) {
+                    constexpr int k = KC;
+                    constexpr int e = E0 + k;
+                    constexpr int j = S::T.ecol[e];
+                    if constexpr (j < NND) {
+                        const uint32_t addr = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                        __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, (uint32_t)(int)dtv[k], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    } else {
+                        float& ap = app1[S::T.d1_slot[j]];
+                        ap = ap + dtv[k];
+                    }
+                });
+                return;
+            }
             // VN update, ascending block row per column
 #if NRLDPC_LDS_ATOMIC_ACC
             static_for<0, D>([&](auto KC) {
@@ -535,7 +556,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         if (!idle) {
 #pragma unroll 2
             for (int j = 0; j < NND; j++) {
-                float v = lds_ld(rAcc + j * Z4);
+                float v;
+                if constexpr (CHAR) v = (float)(int)lds_ldu(rAcc + j * Z4);
+                else v = lds_ld(rAcc + j * Z4);
                 if constexpr (HALF) {  // the APP between iterations is stored in binary16
                     if (!last) {
                         v = round_half(v);
@@ -544,7 +567,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 } else if constexpr (CHAR) {  // ... or as a signed char
                     if (!last) {
                         v = __builtin_amdgcn_fmed3f(v, -128.0f, 127.0f);
-                        lds_st(rAcc + j * Z4, v);
+                        lds_stu(rAcc + j * Z4, (uint32_t)(int)v);
                     }
                 }
                 lds_st(rRd + j * 2 * Z4, v);
