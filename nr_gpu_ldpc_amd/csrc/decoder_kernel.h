@@ -167,7 +167,7 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // TRAIN (learned only): also store, per iteration and expanded row, the two minima, the
 // first-strict-minimum and second-minimum edge indices and the Q sign bits, for the
 // backward pass (layout [cw][iteration][row][Z]).
-// MIXED (persistent mixed-batch kernel, ZC == 0): the workgroup decodes one tile of a mixed
+// MIXED (persistent mixed-batch kernel, any ZC): the workgroup decodes one tile of a mixed
 // batch.  `mcw` lists the tile's codewords; each has its own parity-row count (at most
 // a.m_rows, the tile's maximum), length and output location.  Rows beyond a codeword's own
 // count are computed with zero channel LLRs on their extension columns and their VN updates
@@ -193,7 +193,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                                              const bool emit_only_ok = false) {
     using S = Structure<KIND>;
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
-    static_assert(!(MIXED && (TRAIN || ZC > 0)), "mixed tiles use the any-Z plain/learned path");
+    static_assert(!(MIXED && TRAIN), "mixed tiles use the plain/learned forward path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
     static_assert(!(CHAR && (LEARNED || TRAIN || MIXED || HALF)), "the signed-char schedule is plain min-sum");
     constexpr int ROWS = S::ROWS;

@@ -3,13 +3,32 @@
 // The host planner (mixed_plan.cpp) groups the batch into tiles of one (base graph, Z) each, at
 // most cpb codewords per tile, ordered longest first.  A grid sized to the resident
 // workgroups pulls tiles from a device work counter until the list is exhausted; every tile
-// runs the any-Z fused decoder body with per-codeword row counts, early exit and outputs.
+// runs the fused decoder body with per-codeword row counts, early exit and outputs (compiled-in
+// offsets for BG1 Z=384 and BG2 Z=208, the per-edge table for every other size).
 // The last workgroup to finish clears the counters, so consecutive launches (and graph
 // replays) need no reset.
 #include "status.h"
 #include "decoder_kernel.h"
 
 namespace nrldpc {
+
+using CTile = const MixedTile __attribute__((address_space(4)));
+
+// One tile of (KIND, ZC): ZC > 0 is a lifting size with every circulant offset compiled in.
+template <int KIND, int ZC, bool LEARNED>
+__device__ __forceinline__ void mixed_tile(const DecodeArgs& a, const MixedCw* cws, CTile* t, bool fb,
+                                           uint32_t slot) {
+    if (LEARNED && t->trained) {
+        // learned weights; with fallback, the syndrome of the final hard decision decides which
+        // codewords plain min-sum decodes again (its output replaces the learned one only where
+        // it passes the check)
+        const bool ok = decode_block<KIND, ZC, true, false, true>(a, 0, cws, true, fb, false);
+        const bool need = fb && (int)threadIdx.x < t->count * a.Z && !ok;
+        if (fb && block_any(need, slot)) decode_block<KIND, ZC, false, false, true>(a, 0, cws, need, true, true);
+    } else {
+        decode_block<KIND, ZC, false, false, true>(a, 0, cws);
+    }
+}
 
 template <bool LEARNED>
 __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
@@ -23,7 +42,6 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         if (ti >= n_tiles) break;
         // Tile fields are wave-uniform: read them through the constant address space so that
         // they land in SGPRs (scalar loads).
-        using CTile = const MixedTile __attribute__((address_space(4)));
         CTile* t = (CTile*)(m.tiles) + ti;
         const int kind = t->kind;
         DecodeArgs a{};
@@ -42,22 +60,15 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         a.lds_cw_bytes = t->lds_cw_bytes;
         a.nnz = kind == kBG1 ? Structure<kBG1>::NNZ : Structure<kBG2>::NNZ;
         const MixedCw* cws = m.cws + t->first;
-        if (LEARNED && t->trained) {
-            // learned weights; with fallback, the syndrome of the final hard decision decides
-            // which codewords plain min-sum decodes again (its output replaces the learned one
-            // only where it passes the check)
-            const bool fb = m.fallback != 0;
-            bool ok;
-            if (kind == kBG1) ok = decode_block<kBG1, 0, true, false, true>(a, 0, cws, true, fb, false);
-            else ok = decode_block<kBG2, 0, true, false, true>(a, 0, cws, true, fb, false);
-            const bool need = fb && (int)threadIdx.x < t->count * a.Z && !ok;
-            if (fb && block_any(need, slot)) {
-                if (kind == kBG1) decode_block<kBG1, 0, false, false, true>(a, 0, cws, need, true, true);
-                else decode_block<kBG2, 0, false, false, true>(a, 0, cws, need, true, true);
-            }
+        const bool fb = m.fallback != 0;
+        // the two lifting sizes of the NR configurations that carry most traffic (BG1 Z=384,
+        // BG2 Z=208) run with compiled-in offsets, every other one with the per-edge table
+        if (kind == kBG1) {
+            if (a.Z == 384) mixed_tile<kBG1, 384, LEARNED>(a, cws, t, fb, slot);
+            else mixed_tile<kBG1, 0, LEARNED>(a, cws, t, fb, slot);
         } else {
-            if (kind == kBG1) decode_block<kBG1, 0, false, false, true>(a, 0, cws);
-            else decode_block<kBG2, 0, false, false, true>(a, 0, cws);
+            if (a.Z == 208) mixed_tile<kBG2, 208, LEARNED>(a, cws, t, fb, slot);
+            else mixed_tile<kBG2, 0, LEARNED>(a, cws, t, fb, slot);
         }
     }
     if (threadIdx.x == 0) {
