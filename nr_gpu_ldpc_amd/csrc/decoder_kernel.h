@@ -79,7 +79,8 @@ __device__ __forceinline__ bool block_any(bool v, uint32_t slot) {
     __syncthreads();
     if (threadIdx.x == 0) lds_stu(slot, 0u);
     __syncthreads();
-    if (v) lds_stu(slot, 1u);
+    // one store per wavefront (same-address LDS stores of many lanes are serialised)
+    if (__ballot(v) != 0u && (threadIdx.x & 63u) == 0u) lds_stu(slot, 1u);
     __syncthreads();
     const bool any = lds_ldu(slot) != 0u;
     __syncthreads();
@@ -91,8 +92,10 @@ __device__ __forceinline__ uint32_t block_any2(bool a, bool b, uint32_t slot) {
     __syncthreads();
     if (threadIdx.x == 0) lds_stu(slot, 0u);
     __syncthreads();
-    const uint32_t v = (a ? 1u : 0u) | (b ? 2u : 0u);
-    if (v) __hip_atomic_fetch_or((lds_u32*)(uintptr_t)slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    // reduced over the wavefront first: one atomic per wavefront instead of one per lane
+    const uint32_t v = (__ballot(a) != 0u ? 1u : 0u) | (__ballot(b) != 0u ? 2u : 0u);
+    if (v != 0u && (threadIdx.x & 63u) == 0u)
+        __hip_atomic_fetch_or((lds_u32*)(uintptr_t)slot, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     __syncthreads();
     const uint32_t m = lds_ldu(slot);
     __syncthreads();
@@ -619,8 +622,18 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 if constexpr (MIXED) odd &= __ballot(i < m_lane);
                 badm |= odd;
             });
-            const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-            if ((badm >> lane) & 1u) lds_stu(flagAddr, 1u);
+            // One store per codeword and wavefront, by the codeword's first lane in it, if any of
+            // the codeword's lanes here has an unsatisfied check (lanes of one codeword are
+            // contiguous; dead lanes lie beyond the codeword they shadow).
+            {
+                int lane = tid & 63;
+                asm volatile("" : "+v"(lane));  // computed here, not kept live across the loop
+                const int wb = tid - lane;
+                const int lo = max(cwl * Z - wb, 0);
+                const int hi = min((cwl + 1) * Z - wb, 64);
+                const uint64_t segm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
+                if (lane == lo && (badm & segm) != 0u) lds_stu(flagAddr, 1u);
+            }
             __syncthreads();
             const bool conv = lds_ldu(flagAddr) == 0u;
             if (fin) {
