@@ -4,8 +4,9 @@
 * The Gaussian stream by its moments.
 * The decoder against hand-derived closed forms and against an independent dense-matrix
   numpy restatement of the reference kernels (first-strict-minimum index rule, float32).
-* The learned decoder: alpha=0.75/beta=0 is the plain decoder bit for bit; the analytic
-  gradient agrees with double-precision finite differences.
+* The learned decoder: alpha=0.75/beta=0 is the plain decoder bit for bit; general weights
+  equal the dense restatement; the analytic gradient agrees with double-precision finite
+  differences.
 """
 import ctypes
 
@@ -59,11 +60,13 @@ def _dense_H(oracle, code):
     return H
 
 
-def dense_minsum(H, llr, n_iter, half=False, char=False):
+def dense_minsum(H, llr, n_iter, half=False, char=False, alpha=None, beta=None, eidx=None):
     """Flooding scaled min-sum on a dense H, written from the reference kernels' arithmetic.
     half: the HALF_PRC build (R, dt and the APP between iterations rounded to binary16).
     char: the CHAR_PRC build (R and dt truncated to signed char, saturating; the APP between
     iterations clamped to [-128, 127]; a saturated APP enters the check node as it is).
+    alpha, beta [n_iter][n_base_edges], eidx [M][N] (base edge of each nonzero of H): the learned
+    rule R = sign * alpha * max(m - beta, 0) of the iteration's weights.
     Returns hard decisions, APP and, for char, the number of out-of-range conversions."""
     M, N = H.shape
     st = (lambda v: np.asarray(v, F).astype(np.float16).astype(F)) if half else (lambda v: np.asarray(v, F))
@@ -92,7 +95,13 @@ def dense_minsum(H, llr, n_iter, half=False, char=False):
             sq = np.where(Q < 0, F(-1.0), F(1.0))
             sign = F(np.prod(sq))
             mag = np.where(np.arange(len(cols)) == idx, m2, m1).astype(F)
-            Rn = ((F(0.75) * sign) * sq * mag).astype(F)
+            if alpha is None:
+                Rn = ((F(0.75) * sign) * sq * mag).astype(F)
+            else:
+                b = eidx[m, cols]
+                off = (mag - beta[t, b].astype(F)).astype(F)
+                off = np.where(off > 0, off, F(0.0)).astype(F)
+                Rn = (sign * sq * (alpha[t, b].astype(F) * off).astype(F)).astype(F)
             if char:
                 Rc = to_char(Rn)
                 dt[m, cols] = Rc if t == 0 else to_char(Rc - R[m, cols])
@@ -192,6 +201,38 @@ def test_noms_plain_weights_equal_plain_decoder(oracle, kind, Z):
     hd, app = oracle.decode_noms(code, llr, T, a, b)
     phd, papp, _ = oracle.decode(code, llr, T)
     assert np.array_equal(hd, phd) and np.array_equal(app.view(np.uint32), papp.view(np.uint32))
+
+
+@pytest.mark.parametrize("kind,Z,m", [(OR_WIFI, 5, 0), (OR_BG2, 2, 0), (OR_BG1, 3, 0), (OR_BG2, 3, 6)])
+def test_oracle_noms_matches_dense_restatement(oracle, kind, Z, m):
+    """Learned forward with general per-(iteration, edge) weights: the oracle equals the dense
+    restatement bit for bit (and differs from the plain decoder)."""
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    V = oracle.shifts(code)
+    eidx = np.full(H.shape, -1, dtype=np.int64)
+    e = 0
+    for i in range(V.shape[0]):
+        for j in range(V.shape[1]):
+            if V[i, j] >= 0:
+                for r in range(Z):
+                    eidx[i * Z + r, j * Z + (r + V[i, j]) % Z] = e
+                e += 1
+    T = 5
+    rng = np.random.default_rng(Z * 7 + kind)
+    alpha = rng.uniform(0.5, 1.0, (T, e)).astype(F)
+    beta = rng.uniform(0.0, 0.6, (T, e)).astype(F)
+    n = H.shape[1]
+    llr = (rng.standard_normal((3, n)) * 2.5 + 1.0).astype(F)
+    if kind != OR_WIFI:
+        llr[:, :2 * Z] = 0
+    hd, app = oracle.decode_noms(code, llr, T, alpha.reshape(-1), beta.reshape(-1))
+    for w in range(3):
+        dhd, dapp = dense_minsum(H, llr[w], T, alpha=alpha, beta=beta, eidx=eidx)
+        assert np.array_equal(hd[w], dhd)
+        assert np.array_equal(app[w].view(np.uint32), dapp.view(np.uint32))
+    _, papp, _ = oracle.decode(code, llr, T)
+    assert not np.array_equal(papp, app)
 
 
 def test_noms_gradient_matches_finite_differences(oracle, nr):
