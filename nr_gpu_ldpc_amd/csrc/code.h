@@ -125,6 +125,12 @@ template <int KIND> struct Structure {
     static constexpr int MAXDEG = T.max_deg;
     static constexpr int NW = T.n_w;
     static constexpr bool half(int i) { return deg(i) <= 11; }
+    // edges of row i whose column is LDS-resident (degree > 1)
+    static constexpr int row_lds_edges(int i) {
+        int n = 0;
+        for (int e = T.row_start[i]; e < T.row_start[i + 1]; e++) n += T.nd_index[T.ecol[e]] >= 0;
+        return n;
+    }
 };
 
 // Every LDS-resident column lies in the information + core part, which parity-row truncation

@@ -612,50 +612,59 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         }
         const bool check = early || fin;
         if (check && r == 0 && !idle) lds_stu(flagAddr, 0u);
+        // the workgroup vote word of this check is cleared here too (the previous check's read
+        // of it is separated from this store by the barrier after the row loop)
+        if (check && !uniform_cw && tid == 0) lds_stu(wgFlag, 0u);
         __syncthreads();
         if (check) {
             // Syndrome of the current hard decision; any unsatisfied check keeps decoding.  The
             // parity of a check row is taken over wavefront ballots of "APP > 0" (hard decision
             // 0): one compare per edge, the XOR of the 64-lane masks on the scalar unit.
-            // A wavefront whose lanes all belong to one codeword stops at its first row with an
-            // unsatisfied check (a uniform branch): before convergence that is almost always
-            // one of the first rows.
+            // A wavefront stops (a uniform branch) once every codeword among its lanes has an
+            // unsatisfied check: before convergence that is almost always one of the first rows.
             uint64_t badm = 0u;
+            // lanes of this wavefront that belong to this lane's codeword: [lo, hi) (lanes of
+            // one codeword are contiguous; dead lanes lie beyond the codeword they shadow)
+            int lane = tid & 63;
+            asm volatile("" : "+v"(lane));  // computed here, not kept live across the loop
+            const int wb = tid - lane;
+            const int lo = max(cwl * Z - wb, 0);
+            const int hi = min((cwl + 1) * Z - wb, 64);
+            const uint64_t segm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
+            bool stop = false;
             if (!idle) static_for<0, ROWS>([&](auto IC) {
                 constexpr int i = IC;
                 constexpr int D = S::deg(i);
                 constexpr int E0 = S::T.row_start[i];
                 int mr = m_rows;
                 asm volatile("" : "+s"(mr));
-                if (i >= mr || (wave_one_cw && badm != 0u)) return;
+                if (i >= mr || stop) return;
                 asm volatile("" : "+s"(et));
                 uint64_t zeros = 0u;  // lanes with an odd number of hard-decision zeros
+                float xs[D];
+                constexpr int NLD = S::row_lds_edges(i);
                 static_for<0, D>([&](auto KC) {
                     constexpr int e = E0 + KC;
                     constexpr int j = S::T.ecol[e];
-                    float x;
-                    if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
-                    else x = app1[S::T.d1_slot[j]];
-                    zeros ^= __ballot(x > 0.f);
+                    if constexpr (j < NND) xs[KC] = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    else xs[KC] = app1[S::T.d1_slot[j]];
                 });
+                // All of the row's LDS reads are issued before the first compare: under the
+                // kernel's register pressure the scheduler would otherwise wait for each read
+                // before issuing the next.
+                if constexpr (NLD > 0) __builtin_amdgcn_sched_group_barrier(0x100, NLD, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, 2 * D, 0);
+                static_for<0, D>([&](auto KC) { zeros ^= __ballot(xs[KC] > 0.f); });
                 // ones = D - zeros: odd parity of the ones where that of the zeros differs from D's
                 // (complemented masks are cut to the active lanes of a partial last wavefront)
                 uint64_t odd = (D & 1) ? (~zeros & __ballot(true)) : zeros;
                 if constexpr (MIXED) odd &= __ballot(i < m_lane);
                 badm |= odd;
+                if (badm != 0u) stop = wave_one_cw || __ballot((badm & segm) == 0u) == 0u;
             });
             // One store per codeword and wavefront, by the codeword's first lane in it, if any of
-            // the codeword's lanes here has an unsatisfied check (lanes of one codeword are
-            // contiguous; dead lanes lie beyond the codeword they shadow).
-            {
-                int lane = tid & 63;
-                asm volatile("" : "+v"(lane));  // computed here, not kept live across the loop
-                const int wb = tid - lane;
-                const int lo = max(cwl * Z - wb, 0);
-                const int hi = min((cwl + 1) * Z - wb, 64);
-                const uint64_t segm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
-                if (lane == lo && (badm & segm) != 0u) lds_stu(flagAddr, 1u);
-            }
+            // the codeword's lanes here has an unsatisfied check.
+            if (lane == lo && (badm & segm) != 0u) lds_stu(flagAddr, 1u);
             __syncthreads();
             const bool conv = lds_ldu(flagAddr) == 0u;
             if (fin) {
@@ -669,7 +678,12 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 any_fresh = __builtin_amdgcn_readfirstlane((int)fresh) != 0;
                 any_rest = __builtin_amdgcn_readfirstlane((int)rest) != 0;
             } else {
-                const uint32_t m = block_any2(fresh, rest, wgFlag);
+                // one atomic per wavefront into the vote word cleared before the check
+                const uint32_t v = (__ballot(fresh) != 0u ? 1u : 0u) | (__ballot(rest) != 0u ? 2u : 0u);
+                if (v != 0u && (tid & 63) == 0)
+                    __hip_atomic_fetch_or((lds_u32*)(uintptr_t)wgFlag, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                __syncthreads();
+                const uint32_t m = lds_ldu(wgFlag);
                 any_fresh = (m & 1u) != 0u;
                 any_rest = (m & 2u) != 0u;
             }
