@@ -362,13 +362,14 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         __syncthreads();
     };
 
-    // Compiled-in offsets (ZC > 0, single-code launches): the read-copy values of row i+1 are
-    // loaded while row i updates the accumulator, so that their LDS latency overlaps that work
-    // instead of stalling the start of row i+1 (the read copy is not written during an
-    // iteration).  Measured at BG1 Z=384: CHAR 5.15 -> 4.90 ms per 4096 codewords, plain FP32
-    // -0.7 %; the learned kernel (scalar weight loads in the same window) is 1 % slower with
-    // it and keeps the in-row reads.
-    constexpr bool PF = ZC > 0 && !MIXED && !TRAIN && !LEARNED;
+    // CHAR schedule with compiled-in offsets (ZC > 0, single-code launches): the read-copy
+    // values of row i+1 are loaded while row i adds into the accumulator, so that their LDS
+    // latency overlaps that work instead of stalling the start of row i+1 (the read copy is not
+    // written during an iteration).  Measured at BG1 Z=384: 5.15 -> 4.90 ms per 4096
+    // codewords.  The barrier-ordered schedules gain at most 1 % from it (FP32 plain, binary16)
+    // or lose 1 % (learned), and it pushes their kernels into register spills, so they keep
+    // the in-row reads.
+    constexpr bool PF = CHAR && ZC > 0 && !MIXED;
     float xp[S::T.max_deg > 0 ? S::T.max_deg : 1];
     auto prefetch = [&](auto IC) {
         constexpr int i = IC;
