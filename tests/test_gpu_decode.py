@@ -99,6 +99,31 @@ def test_early_exit_matches_oracle(nr, oracle, torch_gpu, kind, Z, cpb, n_cw, eb
     assert iters.min() < 20
 
 
+@pytest.mark.parametrize("Z,cpb", [(24, 2), (13, 4), (52, 3)])
+def test_early_exit_mixed_convergence_in_one_wavefront(nr, oracle, torch_gpu, Z, cpb):
+    """Codewords that converge at once (noiseless), late (moderate noise) and never (pure
+    noise) side by side in one workgroup and wavefront: iteration counts, decisions and APP equal
+    the oracle's (the syndrome walk of a wavefront stops only when every codeword in it has a
+    failing row)."""
+    print(f"case early exit mixed convergence BG2 Z={Z} cpb={cpb}", flush=True)
+    code = nr.Code(2, Z)
+    code.set_launch(cpb)
+    n = 4 * cpb
+    good, _ = _noisy_llr(nr, code, n, 40.0, 0.2, seed=Z)
+    mid, _ = _noisy_llr(nr, code, n, 1.5, 0.2, seed=Z + 1)
+    rng = np.random.default_rng(Z + 2)
+    bad = (rng.standard_normal((n, code.n_vn)) * 3).astype(np.float32)
+    bad[:, :code.punct * Z] = 0
+    pick = rng.integers(0, 3, n)
+    pick[:3] = [0, 2, 1]
+    llr = np.where((pick == 0)[:, None], good, np.where((pick == 1)[:, None], mid, bad)).astype(np.float32)
+    hd, app, iters = _gpu_decode(torch_gpu, code, llr, 20, flags=nr.EARLY_EXIT)
+    ohd, oapp, oit = oracle.decode(oracle.code(OR_BG2, Z), llr, 20, early_exit=True)
+    assert np.array_equal(iters, oit)
+    assert np.array_equal(hd, ohd) and np.array_equal(app, oapp)
+    assert iters.min() <= 2 and iters.max() == 20
+
+
 def test_special_values(nr, oracle, torch_gpu):
     """Signed zeros, exact ties and magnitudes above the 1000.0f start value."""
     print("case special values QC12x24 Z=8", flush=True)

@@ -70,6 +70,24 @@ def test_mixed_matches_single_code_decoder(nr, torch_gpu, early):
         assert got[2].min() < 12
 
 
+@pytest.mark.parametrize("early", [True, False])
+def test_large_mixed_batch_matches_single_code_decoder(nr, torch_gpu, early):
+    """8192 codewords at Z 40..104 (tiles of 2 to 12 codewords that converge at different
+    iterations): every codeword equals the single-code decoder bit for bit, iteration counts
+    included."""
+    torch = torch_gpu
+    print(f"case mixed 8192 Z 40..104 early={early}", flush=True)
+    b = mx.draw_batch(8192, seed=41, sizes=[z for z in mx.LIFTING_SIZES if 40 <= z <= 104], ebn0=(0.0, 3.0))
+    llr, _ = mx.make_channel(torch, b, seed=43)
+    m = nr.Mixed(len(b["kind"]))
+    flags = nr.EARLY_EXIT if early else 0
+    got = _mixed(nr, torch, b, llr, 20, flags, m)
+    want = _reference(nr, torch, b, llr, 20, flags, {})
+    _compare(b, got, want)
+    if early:
+        assert got[2].min() < 20 and got[2].max() == 20
+
+
 def test_mixed_graph_replay_follows_the_plan(nr, torch_gpu):
     """One captured graph (plan upload, LLR copy in, decode, HD copy out) replayed for two
     different plans gives what direct decodes give."""
