@@ -1,0 +1,104 @@
+"""Every TS 38.212 lifting size of both base graphs (102 (BG, Z) pairs) and the 12x24 code, decoded
+on the GPU by each schedule and compared with the oracle bit for bit: plain FP32 (with and without
+early exit, full and truncated parity rows), HALF_PRC, CHAR_PRC and learned offset/normalized
+min-sum with random per-(iteration, edge) weights.  Two or three noisy codewords per pair keep the
+oracle's share to a few seconds.  Each case prints its parameters before it runs."""
+import numpy as np
+import pytest
+
+from tests.oracle_binding import OR_BG1, OR_BG2, OR_WIFI
+from tests.test_gpu_char import _char_decode
+from tests.test_gpu_decode import _gpu_decode, _noisy_llr
+from tests.test_gpu_half import _half_decode
+from tests.test_gpu_learned import _learned
+from tests.test_oracle import _quantize
+
+pytestmark = pytest.mark.gpu
+KIND = {1: OR_BG1, 2: OR_BG2, 3: OR_WIFI}
+LIFTING = sorted({a * 2 ** j for a in (2, 3, 5, 7, 9, 11, 13, 15) for j in range(8) if a * 2 ** j <= 384})
+ROWS = {1: 46, 2: 42}
+KB = {1: 22, 2: 10}
+PAIRS = [(k, z) for k in (1, 2) for z in LIFTING] + [(3, z) for z in (4, 17, 96, 127)]
+
+
+def _case(kind, Z, salt):
+    """Parity rows and nominal rate of one case: every third pair keeps a random row count."""
+    rng = np.random.default_rng(1000 * kind + Z + salt)
+    if kind == 3:
+        return 0, 0.5
+    m = int(rng.integers(4, ROWS[kind] + 1)) if (Z + salt) % 3 == 0 else 0
+    rows = m or ROWS[kind]
+    return m, KB[kind] / (KB[kind] + rows - 2.0)
+
+
+def test_lifting_set_is_complete():
+    assert len(LIFTING) == 51 and len(PAIRS) == 106
+
+
+def test_plain_every_lifting_size(nr, oracle, torch_gpu):
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 0)
+        print(f"case plain kind={kind} Z={Z} m={m}", flush=True)
+        code = nr.Code(kind, Z, m)
+        oc = oracle.code(KIND[kind], Z, m)
+        llr, _ = _noisy_llr(nr, code, 3, 1.0, rate, seed=7 * Z + kind)
+        hd, app, it = _gpu_decode(torch_gpu, code, llr, 4)
+        ohd, oapp, _ = oracle.decode(oc, llr, 4)
+        assert np.array_equal(hd, ohd), (kind, Z, m)
+        assert np.array_equal(app, oapp), (kind, Z, m)
+        assert (it == 4).all()
+        # early exit at a higher Eb/N0: iteration counts and outputs equal the oracle's
+        llr, _ = _noisy_llr(nr, code, 3, 3.0, rate, seed=11 * Z + kind)
+        hd, app, it = _gpu_decode(torch_gpu, code, llr, 12, flags=nr.EARLY_EXIT)
+        ohd, oapp, oit = oracle.decode(oc, llr, 12, early_exit=True)
+        assert np.array_equal(it, oit), (kind, Z, m, it, oit)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
+
+
+def test_half_every_lifting_size(nr, oracle, torch_gpu):
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 1)
+        print(f"case half kind={kind} Z={Z} m={m}", flush=True)
+        code = nr.Code(kind, Z, m)
+        llr, _ = _noisy_llr(nr, code, 2, 1.0, rate, seed=5 * Z + kind)
+        llr_h = llr.astype(np.float16).view(np.uint16)
+        hd, app, _ = _half_decode(torch_gpu, code, llr_h, 4)
+        ohd, oapp, _ = oracle.decode_half(oracle.code(KIND[kind], Z, m), llr_h, 4)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
+
+
+def test_char_every_lifting_size(nr, oracle, torch_gpu):
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 2)
+        gain = np.float32(4.0 if Z % 2 else 1.0)  # odd Z: scaled LLRs, so the saturating rules fire
+        print(f"case char kind={kind} Z={Z} m={m} gain={gain}", flush=True)
+        code = nr.Code(kind, Z, m)
+        llr, _ = _noisy_llr(nr, code, 2, 1.0, rate, seed=3 * Z + kind)
+        q = _quantize(llr * gain)
+        hd, app, _ = _char_decode(torch_gpu, code, q, 4)
+        ohd, oapp, _, _ = oracle.decode_char(oracle.code(KIND[kind], Z, m), q, 4)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
+
+
+def test_learned_every_lifting_size(nr, oracle, torch_gpu):
+    T = 3
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 0)
+        print(f"case learned kind={kind} Z={Z} m={m}", flush=True)
+        code = nr.Code(kind, Z, m)
+        rng = np.random.default_rng(kind * 1000 + Z)
+        w = np.stack([rng.uniform(0.5, 1.0, (T, code.nnz_full)), rng.uniform(-0.3, 0.5, (T, code.nnz_full))], -1)
+        w = w.astype(np.float32)
+        code.set_weights(w)
+        llr, _ = _noisy_llr(nr, code, 2, 1.0, rate, seed=13 * Z + kind)
+        hd, app, _ = _learned(torch_gpu, code, llr, T)
+        # the library keeps weights for every edge of the full base graph; the oracle takes the
+        # kept rows' edges only, which are the leading code.nnz ones (edges are numbered row-major)
+        wk = w[:, :code.nnz, :]
+        ohd, oapp = oracle.decode_noms(oracle.code(KIND[kind], Z, m), llr, T, wk[..., 0].reshape(-1),
+                                       wk[..., 1].reshape(-1))
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
