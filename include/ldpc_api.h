@@ -68,7 +68,8 @@ int ldpc_ctx_create(ldpc_ctx** out, int kind, int Z, int m_rows, int device);
 int ldpc_ctx_destroy(ldpc_ctx* ctx);
 int ldpc_ctx_info(const ldpc_ctx* ctx, ldpc_code_info* info);
 
-/* Decode n_cw codewords with flooding scaled min-sum (alpha = 0.75), n_iter iterations.
+/* Decode n_cw codewords with flooding scaled min-sum (alpha = 0.75), n_iter >= 1 iterations
+ * (LDPC_EINVAL otherwise; n_cw == 0 launches nothing and returns LDPC_OK).
  * Replaces perform_gpu_ldpc_decoding<T>  src/gpu_decoder_top.cu:16-52 and the kernels
  * cnp_kernel_1st_iter / cnp_kernel / vnp_kernel_normal / vnp_kernel_last_iter /
  * pack_hard_decision  src/ldpc_kernels.cu:72-500.

@@ -222,7 +222,7 @@ int ldpc_set_launch(ldpc_ctx* x, int cpb) {
 
 int ldpc_decode_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                       uint8_t* hd_packed, float* app_out, int* iters_out) {
-    if (!x || !llr || !hd_packed || n_cw < 0 || n_iter < 0) return LDPC_EINVAL;
+    if (!x || !llr || !hd_packed || n_cw < 0 || n_iter < 1) return LDPC_EINVAL;
     if (n_cw == 0) return LDPC_OK;
     return decode_with(x, stream, llr, n_cw, n_iter, flags, hd_packed, app_out, iters_out, nullptr, nullptr,
                        nullptr);

@@ -43,6 +43,20 @@ def test_host_only_context_refuses_device_work(nr):
         raise AssertionError("decode on a host-only context must fail")
 
 
+def test_decode_entry_points_refuse_zero_iterations(nr):
+    """Every decode entry point takes n_iter >= 1; the argument check runs before any device work."""
+    code = nr.Code(1, 384, 0, -1)
+    calls = [code.decode, code.decode_half, code.decode_char, code.decode_learned]
+    for call in calls:
+        for n_iter in (0, -1):
+            try:
+                call(1, 1, n_iter, 1)
+            except nr.LdpcError as e:
+                assert e.status == -1, (call.__name__, n_iter, e.status)
+            else:
+                raise AssertionError(f"{call.__name__} accepted n_iter={n_iter}")
+
+
 def test_cpp_host_api_exported(nr):
     """Every function of include/nr_gpu_ldpc.hpp (the reference-named C++ API) is in the library."""
     import shutil
