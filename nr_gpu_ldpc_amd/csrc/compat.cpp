@@ -290,6 +290,10 @@ host_mem<T> setup_host(const params* p, unsigned int n_info_bits, unsigned int n
     h.recv = static_cast<float*>(malloc(h.mem_size_llr));
     h.llr_quan = static_cast<T*>(malloc(n_cw_bits * sizeof(T)));
     const long long n_cw = (long long)p->n_mcw * p->n_cw_per_mcw;
+    // The per-stream byte counts are int fields (as in the reference's host_mem); a batch whose
+    // LLR buffer would not fit one is refused instead of being allocated with a wrapped size.
+    if (n_cw < 1 || n_cw * (long long)n_cw_bits * (long long)sizeof(T) > 0x7fffffffLL)
+        config_error("setup_host: n_mcw * n_cw_per_mcw codewords must be >= 1 and their LLR bytes fit an int");
     h.mem_size_llr_cuda = (int)(n_cw * n_cw_bits * sizeof(T));
     h.mem_size_hd_cuda = (int)(n_cw * n_cw_bits);
     h.mem_size_hd_cuda_packed = (int)((n_cw * n_cw_bits + 7) / 8);

@@ -17,6 +17,15 @@ static h_element g_h2[32][68];  // wider than the largest column degree (30)
 int main(int argc, char** argv) {
     const ldpc_params a = setup_ldpc(8424, 1.0f / 3.0f, 5);
     fill_in_compact_matrices(g_h1, g_h2, &a);
+    if (argc > 1 && argv[1][0] == 'h') {
+        // 100,000 BG1 Z=384 codewords per stream: 10.4 GB of LLRs, beyond the int size fields
+        params p{};
+        p.n_cw_per_mcw = 1000;
+        p.n_mcw = 100;
+        p.n_streams = 1;
+        setup_host<float>(&p, a.K, a.N_before_puncture);
+        return 0;
+    }
     if (argc > 1) {
         // a row list that no longer describes H is refused before any device work
         g_h1[3][5].value = (short)(g_h1[3][5].value + 1);

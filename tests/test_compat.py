@@ -52,6 +52,18 @@ def test_allocate_constant_memory_refuses_a_wrong_list(tmp_path):
     assert "h_compact1 does not describe ldpc->H" in r.stderr
 
 
+def test_setup_host_refuses_a_batch_beyond_the_int_size_fields(tmp_path):
+    """host_mem keeps per-stream byte counts in int fields; a batch whose LLR bytes exceed
+    INT_MAX is refused (message, abort) before any pinned allocation, so this runs without a GPU."""
+    exe = str(tmp_path / "compat_host")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", f"-I{ROOT}/include",
+                    f"{ROOT}/tests/compat/compat_host.cpp", f"-L{LIBDIR}", "-lnrldpc", f"-Wl,-rpath,{LIBDIR}",
+                    "-o", exe], check=True, capture_output=True)
+    r = subprocess.run([exe, "huge"], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "LLR bytes fit an int" in r.stderr
+
+
 def test_setup_ldpc(host_out):
     # BG1 Z=384: K=8448, N=25344, 46 x 68, 26112 decoder columns, row degree 19, column degree 30
     assert host_out["setup_bg1"] == [1, 384, 8448, 25344, 46, 68, 26112, 19, 30, 5, 19, 30]
