@@ -165,8 +165,11 @@ int ldpc_noms_get_weights(ldpc_ctx* ctx, int* n_iter, float* weights);
  * LDPC_FLAG_FALLBACK_PLAIN, codewords whose learned hard decision fails the syndrome check are
  * decoded again with plain min-sum (n_iter iterations); the plain outputs (hard decisions, APP,
  * iterations) replace the learned ones for the codewords whose plain hard decision passes the
- * syndrome check.  Outputs as for ldpc_decode_async; the fallback uses context scratch memory,
- * so fallback calls on one context must not overlap. */
+ * syndrome check.  Outputs as for ldpc_decode_async.  The fallback uses the context's scratch
+ * memory: fallback calls (and ldpc_noms_train_grad_async) on one context run one after another on
+ * the GPU, whatever thread or stream issues them (each call's stream waits for the previous
+ * call's use of the scratch).  Under stream capture that wait is not recorded; graphs that
+ * share one context must then not be launched concurrently. */
 int ldpc_decode_learned_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
                               unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 

@@ -180,6 +180,10 @@ int ldpc_ctx_destroy(ldpc_ctx* x) {
     if (!x) return LDPC_EINVAL;
     if (x->device < 0) { delete x; return LDPC_OK; }
     (void)hipSetDevice(x->device);
+    if (x->scratch_done) {
+        (void)hipEventSynchronize(x->scratch_done);
+        (void)hipEventDestroy(x->scratch_done);
+    }
     void* ptrs[] = {x->d_etab, x->d_row_start, x->d_edge_col, x->d_edge_vm, x->d_step_target,
                     x->d_step_start, x->d_term_col, x->d_term_shift, x->d_weights, x->d_scratch};
     for (void* p : ptrs)

@@ -31,6 +31,7 @@ struct ldpc_ctx {
     // grow-only scratch for internal temporaries (fallback syndrome weights, training HD)
     void* d_scratch = nullptr;
     size_t scratch_bytes = 0;
+    hipEvent_t scratch_done = nullptr;  // recorded after each use of d_scratch
     std::mutex mu;
 };
 

@@ -13,18 +13,54 @@
 
 namespace nrldpc {
 
-// Grow-only device scratch owned by the context (syndrome weights of the fallback path).
-static int ctx_scratch(ldpc_ctx* x, size_t bytes, void** out) {
-    if (x->scratch_bytes < bytes) {
-        if (x->d_scratch) NRLDPC_HIP(hipFree(x->d_scratch));
-        x->d_scratch = nullptr;
-        x->scratch_bytes = 0;
-        NRLDPC_HIP(hipMalloc(&x->d_scratch, bytes));
-        x->scratch_bytes = bytes;
+// Grow-only device scratch owned by the context (fallback syndrome weights and re-decode
+// outputs, the training forward pass's hard decisions).  One use at a time: a ScratchUse holds
+// the context mutex (host threads) while its call enqueues work, makes the call's stream wait
+// for the previous use's last kernel (the event x->scratch_done, so calls on different streams
+// are serialised on the GPU), and records that event on the call's stream when it ends, error
+// paths included.  Growing waits for the previous use to finish before freeing the old buffer.
+// Under stream capture the event is neither waited on nor recorded: the captured graph orders
+// its own nodes, and graphs that share one context's scratch must not run concurrently.
+class ScratchUse {
+  public:
+    ScratchUse(ldpc_ctx* x, hipStream_t s) : x_(x), s_(s), lk_(x->mu) {}
+    ScratchUse(const ScratchUse&) = delete;
+    ScratchUse& operator=(const ScratchUse&) = delete;
+    ~ScratchUse() {
+        if (!began_ || capturing_) return;
+        if (!x_->scratch_done &&
+            hipEventCreateWithFlags(&x_->scratch_done, hipEventDisableTiming) != hipSuccess) {
+            x_->scratch_done = nullptr;
+            return;
+        }
+        (void)hipEventRecord(x_->scratch_done, s_);
     }
-    *out = x->d_scratch;
-    return LDPC_OK;
-}
+    int acquire(size_t bytes, void** out) {
+        if (x_->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+        NRLDPC_HIP(hipSetDevice(x_->device));  // the scratch lives on the context's device
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        NRLDPC_HIP(hipStreamIsCapturing(s_, &cs));
+        capturing_ = cs != hipStreamCaptureStatusNone;
+        began_ = true;
+        if (x_->scratch_done && !capturing_) NRLDPC_HIP(hipStreamWaitEvent(s_, x_->scratch_done, 0));
+        if (x_->scratch_bytes < bytes) {
+            if (x_->scratch_done) NRLDPC_HIP(hipEventSynchronize(x_->scratch_done));
+            if (x_->d_scratch) NRLDPC_HIP(hipFree(x_->d_scratch));
+            x_->d_scratch = nullptr;
+            x_->scratch_bytes = 0;
+            NRLDPC_HIP(hipMalloc(&x_->d_scratch, bytes));
+            x_->scratch_bytes = bytes;
+        }
+        *out = x_->d_scratch;
+        return LDPC_OK;
+    }
+
+  private:
+    ldpc_ctx* x_;
+    hipStream_t s_;
+    std::lock_guard<std::mutex> lk_;
+    bool began_ = false, capturing_ = false;
+};
 
 static size_t records_per_cw(const ldpc_ctx* x, int n_iter) {
     const int rows_full = x->code.rows_full;
@@ -94,7 +130,8 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     const size_t o_app = o_hd + up((size_t)n_cw * c.hd_bytes());
     const size_t total = o_app + (app_out ? (size_t)n_cw * c.n_vn() * sizeof(float) : 0);
     void* scratch = nullptr;
-    int st = ctx_scratch(x, total, &scratch);
+    ScratchUse use(x, static_cast<hipStream_t>(stream));
+    int st = use.acquire(total, &scratch);
     if (st) return st;
     char* sp = static_cast<char*>(scratch);
     int* w1 = reinterpret_cast<int*>(sp);
@@ -167,7 +204,8 @@ int ldpc_noms_train_grad_async(ldpc_ctx* x, void* stream, const float* llr, cons
     a.rec_m2 = rec_m2;
     a.rec_w = rec_w;
     void* hd_scratch = nullptr;
-    int st = ctx_scratch(x, (size_t)n_cw * a.hd_bytes, &hd_scratch);
+    ScratchUse use(x, static_cast<hipStream_t>(stream));
+    int st = use.acquire((size_t)n_cw * a.hd_bytes, &hd_scratch);
     if (st) return st;
     a.hd = static_cast<uint8_t*>(hd_scratch);
     if ((st = decoder_launch_train(c.kind, a, static_cast<hipStream_t>(stream)))) return st;

@@ -84,6 +84,44 @@ def test_gradient_matches_oracle(nr, oracle, torch_gpu, kind, Z, T):
         assert np.abs(got - want).max() <= 2e-3 * scale, (np.abs(got - want).max(), scale)
 
 
+def test_fallback_calls_on_two_streams_share_the_scratch_safely(nr, torch_gpu):
+    """Three fallback decodes on one context issued back to back on two streams, the second one
+    larger (its scratch grows while the first may still run): every output equals the same decode
+    run alone."""
+    torch = torch_gpu
+    print("case fallback on two streams BG2 Z=52", flush=True)
+    code = nr.Code(2, 52)
+    T = 10
+    w = code.plain_weights(T)
+    w[..., 0] = 0.0  # every noisy codeword fails, so the re-decode path does real work
+    code.set_weights(w)
+    sizes = [96, 400, 160]
+    llrs = [_noisy_llr(nr, code, n, 1.0, 0.2, seed=40 + i)[0] for i, n in enumerate(sizes)]
+    lone = [_learned(torch, code, l, T, flags=nr.FALLBACK_PLAIN) for l in llrs]
+    code2 = nr.Code(2, 52)  # a fresh context: its scratch starts empty and grows at call 2
+    code2.set_weights(w)
+    streams = [torch.cuda.Stream(), torch.cuda.Stream(), None]
+    streams[2] = streams[0]
+    outs = []
+    for l, s in zip(llrs, streams):
+        n = l.shape[0]
+        d_llr = torch.from_numpy(l).cuda()
+        d_hd = torch.zeros((n, code2.hd_bytes), dtype=torch.uint8, device="cuda")
+        d_app = torch.zeros((n, code2.n_vn), dtype=torch.float32, device="cuda")
+        d_it = torch.zeros(n, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()  # inputs in place before the calls are issued back to back
+        outs.append((d_llr, d_hd, d_app, d_it, s))
+    for d_llr, d_hd, d_app, d_it, s in outs:
+        code2.decode_learned(d_llr.data_ptr(), d_llr.shape[0], T, d_hd.data_ptr(), d_app.data_ptr(),
+                             d_it.data_ptr(), flags=nr.FALLBACK_PLAIN, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    for (d_llr, d_hd, d_app, d_it, _), (lhd, lapp, lit) in zip(outs, lone):
+        hd = np.unpackbits(d_hd.cpu().numpy(), axis=1, bitorder="little")[:, :code2.n_vn]
+        assert np.array_equal(hd, lhd)
+        assert np.array_equal(d_app.cpu().numpy().view(np.uint32), lapp.view(np.uint32))
+        assert np.array_equal(d_it.cpu().numpy(), lit)
+
+
 def test_fallback_to_plain(nr, oracle, torch_gpu):
     """alpha = 0 freezes the learned decoder at the channel decision, so every noisy codeword
     fails its syndrome; the plain re-decode replaces it wherever plain min-sum converges to a
