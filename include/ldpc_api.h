@@ -141,6 +141,15 @@ int ldpc_setup_params(int A, float rate, int* out);
  * the last 20 parity rows unencoded; this encoder solves every kept row). */
 int ldpc_encode_host(const ldpc_ctx* ctx, const uint8_t* info, int n_cw, uint8_t* cw);
 
+/* The same systematic encoder on the device: info (device, [n_cw][K] bits, one byte each; only
+ * bit 0 of each byte is read) -> cw (device, [n_cw][n_vn] bits, one byte each), equal to
+ * ldpc_encode_host's output.  One workgroup per codeword runs the context's encoding program
+ * (the closed form of the quasi-cyclic parity equations) in LDS.
+ * Replaces the encoding step of transmit  src/encoding_tools.cpp:67-83, i.e. ldpc_encoder_orig
+ * src/nr_ldpc.cpp:655-913 with the generator shifts of src/Gen_shift_value.h, run per codeword on
+ * the host by generate_new_codewords  src/setup_exp.cu:168-196. */
+int ldpc_encode_async(ldpc_ctx* ctx, void* stream, const uint8_t* info, int n_cw, uint8_t* cw);
+
 /* Generator shift lists regenerated from H for (kind, Z): fills no_shift[rows*kb] and
  * pointer_shift[rows*kb] (as the OAI tables lay them out) and returns the number of shift
  * values written to gen_shift (at most gen_len), or a negative status.

@@ -86,6 +86,7 @@ _sig("ldpc_llr_to_half_async", ctypes.c_int, _P, _P, ctypes.c_size_t, _P)
 _sig("ldpc_base_matrix", ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, ctypes.c_int)
 _sig("ldpc_setup_params", ctypes.c_int, ctypes.c_int, ctypes.c_float, _c_int_p)
 _sig("ldpc_encode_host", ctypes.c_int, _P, _P, ctypes.c_int, _P)
+_sig("ldpc_encode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_generator_shifts", ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, ctypes.c_int)
 _sig("ldpc_noise_stddev", ctypes.c_float, ctypes.c_float, ctypes.c_float)
 _sig("ldpc_channel_async", ctypes.c_int, _P, _P, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, _P,
@@ -280,6 +281,10 @@ class Code:
 
     def adam(self, grad_ptr, m_ptr, v_ptr, step, lr, stream=0):
         _check(lib.ldpc_noms_adam_async(self._h, stream, grad_ptr, m_ptr, v_ptr, step, lr), "ldpc_noms_adam_async")
+
+    def encode(self, info_ptr, n_cw, cw_ptr, stream=0):
+        """Device encoder: info bits [n_cw][K] (uint8, device) -> codeword bits [n_cw][n_vn]."""
+        _check(lib.ldpc_encode_async(self._h, stream, info_ptr, n_cw, cw_ptr), "ldpc_encode_async")
 
     def encode_host(self, info_bits):
         """info_bits: uint8 array [n_cw, K] -> codeword bits uint8 [n_cw, n_vn] (host)."""

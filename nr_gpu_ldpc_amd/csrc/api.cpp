@@ -376,6 +376,28 @@ int ldpc_channel_async(ldpc_ctx* x, void* stream, uint64_t seed, uint64_t stream
     return channel_launch(a, n_cw, static_cast<hipStream_t>(stream));
 }
 
+int ldpc_encode_async(ldpc_ctx* x, void* stream, const uint8_t* info, int n_cw, uint8_t* cw) {
+    if (!x || !info || !cw || n_cw < 0) return LDPC_EINVAL;
+    if (n_cw == 0) return LDPC_OK;
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    const CodeDesc& c = x->code;
+    ChannelArgs a{};
+    a.Z = c.Z;
+    a.n_vn = c.n_vn();
+    a.K = c.K();
+    a.punct = c.punct;
+    a.n_steps = (int)x->prog.step_target.size();
+    a.step_target = x->d_step_target;
+    a.step_start = x->d_step_start;
+    a.term_col = x->d_term_col;
+    a.term_shift = x->d_term_shift;
+    a.info_in = info;
+    a.cw_bits = cw;
+    a.llr = nullptr;
+    return channel_launch(a, n_cw, static_cast<hipStream_t>(stream));
+}
+
 int ldpc_count_errors_async(ldpc_ctx* x, void* stream, const uint8_t* hd, const uint8_t* cw_bits, int n_cw,
                             long long* counts) {
     if (!x || !hd || !cw_bits || !counts || n_cw < 0) return LDPC_EINVAL;

@@ -17,8 +17,9 @@ struct ChannelArgs {
     const int* term_shift;
     const float* sigma_dev;  // [n_cw] or null
     float sigma_all;
+    const uint8_t* info_in;  // [n_cw][K] information bits, one byte each, or null: Philox bits
     uint8_t* cw_bits;        // [n_cw][n_vn] or null
-    float* llr;              // [n_cw][n_vn]
+    float* llr;              // [n_cw][n_vn], or null: encode only (no channel)
 };
 
 struct SyndromeArgs {

@@ -118,7 +118,8 @@ __device__ __forceinline__ void gauss4(uint64_t seed, uint64_t stream, uint64_t 
 
 extern __shared__ __attribute__((aligned(16))) unsigned char c_smem[];
 
-// One workgroup per codeword: info bits -> encoding program -> BPSK + AWGN -> LLR.
+// One workgroup per codeword: info bits (Philox, or the caller's) -> encoding program ->
+// codeword bits (optional) -> BPSK + AWGN -> LLR (skipped when a.llr is null).
 __global__ void __launch_bounds__(256) k_channel(ChannelArgs a) {
     const int cw = blockIdx.x;
     const int Z = a.Z, nvn = a.n_vn, K = a.K;
@@ -126,8 +127,12 @@ __global__ void __launch_bounds__(256) k_channel(ChannelArgs a) {
     uint8_t* bits = c_smem;
     for (int n = threadIdx.x; n < nvn; n += blockDim.x) bits[n] = 0;
     __syncthreads();
+    if (a.info_in) {
+        const uint8_t* in = a.info_in + (size_t)cw * K;
+        for (int n = threadIdx.x; n < K; n += blockDim.x) bits[n] = in[n] & 1u;
+    }
     // information bits: word w of the stream gives bits 32w .. 32w+31 (LSB first)
-    for (int w = threadIdx.x; w < (K + 31) / 32; w += blockDim.x) {
+    for (int w = threadIdx.x; !a.info_in && w < (K + 31) / 32; w += blockDim.x) {
         uint32_t o[4];
         philox_block(a.seed, stream, (uint64_t)(w >> 2), kTagBits, o);
         const uint32_t word = o[w & 3];
@@ -148,12 +153,13 @@ __global__ void __launch_bounds__(256) k_channel(ChannelArgs a) {
         }
         __syncthreads();
     }
+    if (a.cw_bits)
+        for (int n = threadIdx.x; n < nvn; n += blockDim.x) a.cw_bits[(size_t)cw * nvn + n] = bits[n];
+    if (!a.llr) return;  // encode only (no barrier follows)
     const float sigma = a.sigma_dev ? a.sigma_dev[cw] : a.sigma_all;
     const float ss = sigma * sigma;
     const int p0 = a.punct * Z;
     float* llr = a.llr + (size_t)cw * nvn;
-    if (a.cw_bits)
-        for (int n = threadIdx.x; n < nvn; n += blockDim.x) a.cw_bits[(size_t)cw * nvn + n] = bits[n];
     for (int n = threadIdx.x; n < p0; n += blockDim.x) llr[n] = 0.0f;
     // transmitted symbol i = n - p0 uses noise sample i
     const int ntx = nvn - p0;

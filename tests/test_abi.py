@@ -57,6 +57,16 @@ def test_decode_entry_points_refuse_zero_iterations(nr):
                 raise AssertionError(f"{call.__name__} accepted n_iter={n_iter}")
 
 
+def test_device_encoder_argument_checks(nr):
+    """ldpc_encode_async: NULL buffers are refused, a host-only context cannot run it, and an
+    empty batch is a no-op; all decided before any device work."""
+    code = nr.Code(2, 52, 0, -1)
+    assert nr.lib.ldpc_encode_async(code.handle, None, None, 1, ctypes.c_void_p(1)) == -1
+    assert nr.lib.ldpc_encode_async(code.handle, None, ctypes.c_void_p(1), -1, ctypes.c_void_p(1)) == -1
+    assert nr.lib.ldpc_encode_async(code.handle, None, ctypes.c_void_p(1), 0, ctypes.c_void_p(1)) == 0
+    assert nr.lib.ldpc_encode_async(code.handle, None, ctypes.c_void_p(1), 1, ctypes.c_void_p(1)) == -4
+
+
 def test_cpp_host_api_exported(nr):
     """Every function of include/nr_gpu_ldpc.hpp (the reference-named C++ API) is in the library."""
     import shutil

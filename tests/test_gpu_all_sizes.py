@@ -1,8 +1,9 @@
 """Every TS 38.212 lifting size of both base graphs (102 (BG, Z) pairs) and the 12x24 code, decoded
 on the GPU by each schedule and compared with the oracle bit for bit: plain FP32 (with and without
 early exit, full and truncated parity rows), HALF_PRC, CHAR_PRC and learned offset/normalized
-min-sum with random per-(iteration, edge) weights.  Two or three noisy codewords per pair keep the
-oracle's share to a few seconds.  Each case prints its parameters before it runs."""
+min-sum with random per-(iteration, edge) weights; and the device encoder against the host one.
+Two or three noisy codewords per pair keep the oracle's share to a few seconds.  Each case prints
+its parameters before it runs."""
 import numpy as np
 import pytest
 
@@ -101,4 +102,26 @@ def test_learned_every_lifting_size(nr, oracle, torch_gpu):
         ohd, oapp = oracle.decode_noms(oracle.code(KIND[kind], Z, m), llr, T, wk[..., 0].reshape(-1),
                                        wk[..., 1].reshape(-1))
         assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
+
+
+def test_device_encoder_every_lifting_size(nr, oracle, torch_gpu):
+    """ldpc_encode_async equals the host encoder on every pair; only bit 0 of an info byte counts;
+    the codewords satisfy every kept parity check (oracle syndrome)."""
+    torch = torch_gpu
+    for kind, Z in PAIRS:
+        m, _ = _case(kind, Z, 3)
+        print(f"case encode kind={kind} Z={Z} m={m}", flush=True)
+        code = nr.Code(kind, Z, m)
+        rng = np.random.default_rng(17 * Z + kind)
+        raw = rng.integers(0, 256, (5, code.K), dtype=np.uint8)  # any byte; bit 0 is the bit
+        want = code.encode_host(raw & 1)
+        d_info = torch.from_numpy(raw).cuda()
+        d_cw = torch.full((5, code.n_vn), 7, dtype=torch.uint8, device="cuda")
+        code.encode(d_info.data_ptr(), 5, d_cw.data_ptr(), stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        got = d_cw.cpu().numpy()
+        assert np.array_equal(got, want), (kind, Z, m)
+        oc = oracle.code(KIND[kind], Z, m)
+        assert all(oracle.syndrome(oc, h) == 0 for h in got), (kind, Z, m)
         code.close()
