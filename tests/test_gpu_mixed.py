@@ -145,7 +145,11 @@ def test_concurrent_streams_match_lone_decodes(nr, torch_gpu):
         m.capture(s.cuda_stream, d_llr.data_ptr(), 10, d_hd.data_ptr(), 0, d_it.data_ptr(), flags=nr.EARLY_EXIT,
                   llr_host_ptr=h_llr.data_ptr(), llr_floats=b["llr_floats"], hd_host_ptr=h_hd.data_ptr(),
                   hd_bytes=b["hd_total"])
-        slots.append((b, m, s, h_hd, d_it, want, d_llr))
+        # every buffer the captured graph reads or writes stays referenced until the end: a
+        # tensor dropped here would go back to torch's caching allocators and be handed to the next
+        # slot while this graph still copies into or out of it
+        slots.append((b, m, s, h_hd, d_it, want, (d_llr, d_hd, h_llr)))
+    torch.cuda.synchronize()  # the zero fills above ran on the default stream; replays use others
     for rep in range(12):
         b, m, s = slots[rep % 3][:3]
         m.plan(b["desc"])
