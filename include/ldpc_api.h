@@ -270,8 +270,11 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
  * most hd_bytes); the two batch copies are stream copies (DMA engines) so that replays on
  * different streams overlap them with each other's decodes.  Host buffers must be pinned
  * (hipHostMalloc).  Each replay decodes the plan current at that time, into the same buffers,
- * and cuts the copies to the LLR floats and hard-decision bytes that plan addresses.  `stream`
- * is unused (the graph runs on the replay's stream). */
+ * and cuts the copies to the LLR floats and hard-decision bytes that plan addresses.  The graph
+ * keeps the buffer addresses given here: every one of them (llr, hd, app_out, iters_out,
+ * llr_host, hd_host) must stay allocated, and be used for nothing else while a replay runs, until
+ * the last replay has finished or p is destroyed.  `stream` is unused (the graph runs on the
+ * replay's stream). */
 int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
                        float* app_out, int* iters_out, const float* llr_host, size_t llr_floats,
                        uint8_t* hd_host, size_t hd_bytes);
