@@ -636,6 +636,48 @@ def bench_operating_point(torch, nr, args):
     return out
 
 
+def bench_ebn0_sweep(torch, nr, args):
+    """BER/FER of the headline workload (BG1 Z=384, 10 fixed iterations, batches of 4096) over
+    Eb/N0 0..3 dB in 0.25 dB steps: plain min-sum and the learned decoder with plain fallback on
+    the same LLRs, errors counted over the first K decoded bits as the reference counts them."""
+    from nr_gpu_ldpc_amd import noms
+    path = noms.weights_path(nr.BG1, Z, N_ITER)
+    code = nr.Code(nr.BG1, Z)
+    learned = os.path.exists(path)
+    if learned:
+        code.set_weights(noms.load_weights(path)["weights"])
+    dev = torch.device("cuda:0")
+    s = torch.cuda.current_stream().cuda_stream
+    llr = torch.empty((BATCH, code.n_vn), dtype=torch.float32, device=dev)
+    bits = torch.empty((BATCH, code.n_vn), dtype=torch.uint8, device=dev)
+    hd = torch.zeros((BATCH, code.hd_bytes), dtype=torch.uint8, device=dev)
+    cnt = {m: torch.zeros(2, dtype=torch.int64, device=dev) for m in ("plain", "learned")}
+    batches = 2
+    points = []
+    for i in range(13):
+        ebn0 = 0.25 * i
+        for m in cnt.values():
+            m.zero_()
+        for k in range(batches):
+            code.channel(7000 + 100 * i + k, 0, BATCH, llr.data_ptr(), sigma=nr.noise_stddev(ebn0, RATE),
+                         cw_bits_ptr=bits.data_ptr(), stream=s)
+            code.decode(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), stream=s)
+            code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, cnt["plain"].data_ptr(), stream=s)
+            if learned:
+                code.decode_learned(llr.data_ptr(), BATCH, N_ITER, hd.data_ptr(), flags=nr.FALLBACK_PLAIN, stream=s)
+                code.count_errors(hd.data_ptr(), bits.data_ptr(), BATCH, cnt["learned"].data_ptr(), stream=s)
+        frames = batches * BATCH
+        row = {"ebn0_db": ebn0}
+        for m in (("plain", "learned") if learned else ("plain",)):
+            be, fe = cnt[m].tolist()
+            row[m] = {"ber": be / (frames * code.K), "fer": fe / frames}
+        points.append(row)
+    torch.cuda.synchronize()
+    return {"workload": "BG1 Z=384, 10 fixed iterations, Eb/N0 0..3 dB step 0.25, "
+                        f"{batches} x {BATCH} frames per point; learned = committed weights with plain fallback",
+            "frames_per_point": batches * BATCH, "points": points}
+
+
 def bench_reference_example():
     """The reference's shipped example through the BER/FER driver (12 x 24 code, Z=96, T=5,
     80 codewords x 8 streams per SNR point, SNR 0..10 dB; host-generated data, PCIe included)."""
@@ -699,6 +741,7 @@ def main():
     op = bench_operating_point(torch, nr, args)
     if op:
         out["fer1e-3_operating_point"] = op
+    out["ebn0_sweep"] = bench_ebn0_sweep(torch, nr, args)
     if not args.skip_mixed:
         out["mixed_batch"] = bench_mixed(torch, nr, args)
         out["transport_block"] = bench_transport_block(torch, nr, args)

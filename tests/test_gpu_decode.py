@@ -169,3 +169,36 @@ def test_noiseless_roundtrip_full_batch(nr, torch_gpu):
     assert cnt.tolist() == [0, 0]
     hd = np.unpackbits(d_hd[:64].cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
     assert np.array_equal(hd, d_bits[:64].cpu().numpy())
+
+
+def test_bg1_z384_ebn0_sweep_points_match_oracle(nr, oracle, torch_gpu):
+    """The benchmark's Eb/N0 sweep (BG1 Z=384, 10 fixed iterations, 0..3 dB in 0.25 dB steps,
+    LLRs from the device channel): at every point two codewords decode bit-exact (hard decisions
+    and APP) against the oracle, plain and learned with random weights."""
+    torch = torch_gpu
+    code = nr.Code(1, 384)
+    oc = oracle.code(OR_BG1, 384)
+    T = 10
+    rng = np.random.default_rng(5)
+    w = np.stack([rng.uniform(0.5, 1.0, (T, code.nnz_full)), rng.uniform(-0.3, 0.5, (T, code.nnz_full))], -1)
+    w = w.astype(np.float32)
+    code.set_weights(w)
+    d_llr = torch.empty((2, code.n_vn), dtype=torch.float32, device="cuda")
+    for i in range(13):
+        ebn0 = 0.25 * i
+        print(f"case sweep BG1 Z=384 Eb/N0={ebn0}", flush=True)
+        code.channel(7000 + 100 * i, 0, 2, d_llr.data_ptr(), sigma=nr.noise_stddev(ebn0, 1 / 3),
+                     stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        llr = d_llr.cpu().numpy()
+        hd, app, _ = _gpu_decode(torch, code, llr, T)
+        ohd, oapp, _ = oracle.decode(oc, llr, T)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), ebn0
+        d_hd = torch.zeros((2, code.hd_bytes), dtype=torch.uint8, device="cuda")
+        d_app = torch.zeros((2, code.n_vn), dtype=torch.float32, device="cuda")
+        code.decode_learned(d_llr.data_ptr(), 2, T, d_hd.data_ptr(), d_app.data_ptr(),
+                            stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        lhd = np.unpackbits(d_hd.cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
+        lo_hd, lo_app = oracle.decode_noms(oc, llr, T, w[..., 0].reshape(-1), w[..., 1].reshape(-1))
+        assert np.array_equal(lhd, lo_hd) and np.array_equal(d_app.cpu().numpy(), lo_app), ebn0
