@@ -31,6 +31,8 @@
 //   * Packed hard decisions are exact for every Z (the reference's packer assumes Z % 8 == 0);
 //     host_mem::mem_size_hd_cuda_packed is ceil(bits / 8).
 //   * ldpc_encoder_orig(gen_code = 1) (OAI code generation to files) is not provided.
+//   * perform_gpu_ldpc_decoding with n_iterations < 1 is an error (the reference launches no
+//     kernel then and copies back whatever the device's hard-decision buffer holds).
 //   * setup_host aborts with a message when a stream's LLR bytes exceed INT_MAX (the
 //     host_mem size fields are int, as in the reference, whose sizes wrap there instead).
 #ifndef NRLDPC_NR_GPU_LDPC_HPP
