@@ -156,8 +156,21 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
         constexpr int vm = edge_vm<KIND, ZC>(E);
         constexpr uint32_t off = (uint32_t)(J * ZC + vm) * 4u;
         static_assert(off < 0x10000u, "accumulator offset exceeds the LDS immediate");
-        if constexpr (vm == 0) return rAcc + off;
-        else return (r >= ZC - vm ? rAccW : rAcc) + off;
+        if constexpr (vm == 0) {
+            return rAcc + off;
+        } else if constexpr (ZC - vm > 64) {
+            // The threshold is no inline constant: compare against it as a literal operand
+            // (the compiler would otherwise load it into an SGPR with a scalar move, which costs
+            // the wavefront an issue slot per edge).  Same two VALU instructions, VCC clobbered.
+            uint32_t res;
+            asm("v_cmp_gt_i32_e32 vcc, %1, %2\n\tv_cndmask_b32_e32 %0, %3, %4, vcc"
+                : "=v"(res)
+                : "i"(ZC - vm), "v"(r), "v"(rAccW), "v"(rAcc)
+                : "vcc");
+            return res + off;
+        } else {
+            return (r >= ZC - vm ? rAccW : rAcc) + off;
+        }
     } else {
         const uint32_t acc = et[E].acc;
         return (rHi >= acc ? rAccW : rAcc) + (acc & 0xFFFFu);
