@@ -72,16 +72,18 @@ void build_edge_params(const CodeDesc& c, std::vector<EdgeParam>* out) {
     edge_list(c, &er, &ec);
     const int NND = n_nd(c.kind);
     std::vector<EdgeParam>& et = *out;
-    et.assign(er.size(), EdgeParam{0u, 0u});
+    et.assign(er.size(), EdgeParam{0u, 0u, 0u, 0u});
     for (size_t n = 0; n < er.size(); n++) {
         const int v = c.shift(er[n], ec[n]);
         const int j = ec[n];
         if (j < NND) {
             et[n].off_rd = (uint32_t)((j * 2 * Z + v) * 4);
             et[n].acc = ((uint32_t)(Z - v) << 16) | (uint32_t)((j * Z + v) * 4);
+            et[n].acc_off = (uint32_t)((j * Z + v) * 4);
         } else {
             et[n].off_rd = (uint32_t)v;
             et[n].acc = 0;
+            et[n].acc_off = 0;
         }
     }
 }

@@ -11,14 +11,18 @@ namespace nrldpc {
 constexpr int kMaxBlock = 512;          // threads per workgroup (2 waves per SIMD)
 constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per CU (upper bound; see max_lds_bytes)
 
-// Per base-graph edge, in row-major edge order (two dwords, read through the scalar cache).
+// Per base-graph edge, in row-major edge order (four dwords, read through the scalar cache).
 // For an edge into an LDS-resident column (nd = block column j < NND):
-//   off_rd = (j * 2Z + V) * 4                 byte offset into the read copy
-//   acc    = (Z - V) << 16 | (j * Z + V) * 4  wrap threshold | byte offset into the accumulator
-// For an edge into a degree-1 column: off_rd = V (mod Z), acc = 0.
-struct alignas(8) EdgeParam {
+//   off_rd  = (j * 2Z + V) * 4                 byte offset into the read copy
+//   acc     = (Z - V) << 16 | (j * Z + V) * 4  wrap threshold | byte offset into the accumulator
+//   acc_off = (j * Z + V) * 4                  the offset alone, so that the kernel adds it
+//                                              without a scalar mask per edge
+// For an edge into a degree-1 column: off_rd = V (mod Z), acc = acc_off = 0.
+struct alignas(16) EdgeParam {
     uint32_t off_rd;
     uint32_t acc;
+    uint32_t acc_off;
+    uint32_t reserved;
 };
 
 struct DecodeArgs {

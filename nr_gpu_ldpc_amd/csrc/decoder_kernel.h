@@ -173,7 +173,7 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
         }
     } else {
         const uint32_t acc = et[E].acc;
-        return (rHi >= acc ? rAccW : rAcc) + (acc & 0xFFFFu);
+        return (rHi >= acc ? rAccW : rAcc) + et[E].acc_off;
     }
 }
 
