@@ -257,6 +257,10 @@ int ldpc_mixed_tiles(const ldpc_mixed* p, int* out, int max_tiles, int* order);
 
 /* Decode the planned batch: llr / hd / app_out are device buffers addressed by the descriptors'
  * offsets; iters_out[i] (device, may be NULL) is the iteration count of descriptor i.
+ * Launches of one object (decodes and replays) share its device plan and work counter: each one
+ * waits on the GPU for the previous one, whatever stream either runs on (except on a stream under
+ * capture, where the caller must order them).  Objects are independent: use one per stream for
+ * concurrent decodes.
  * flags: LDPC_FLAG_EARLY_EXIT; LDPC_FLAG_LEARNED (codes with weights set use them, the others
  * plain min-sum); with LDPC_FLAG_LEARNED, LDPC_FLAG_FALLBACK_PLAIN re-decodes, inside the same
  * launch, the learned codewords whose final hard decision fails the syndrome check with plain
@@ -269,8 +273,12 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
  * most llr_floats floats), the graph, and an optional device->host copy of hd into hd_host (at
  * most hd_bytes); the two batch copies are stream copies (DMA engines) so that replays on
  * different streams overlap them with each other's decodes.  Host buffers must be pinned
- * (hipHostMalloc).  Each replay decodes the plan current at that time, into the same buffers,
- * and cuts the copies to the LLR floats and hard-decision bytes that plan addresses.  The graph
+ * (hipHostMalloc).  llr_floats and hd_bytes are the sizes of llr (and of app_out, which has the
+ * same layout) and of hd; they are required with a host buffer and optional (0) without one.
+ * Each replay decodes the plan current at that time, into the same buffers, and cuts the copies
+ * to the LLR floats and hard-decision bytes that plan addresses; it returns LDPC_EINVAL, and
+ * enqueues nothing, when that plan addresses more than the sizes given here or holds learned
+ * weights for another iteration count than n_iter.  The graph
  * keeps the buffer addresses given here: every one of them (llr, hd, app_out, iters_out,
  * llr_host, hd_host) must stay allocated, and be used for nothing else while a replay runs, until
  * the last replay has finished or p is destroyed.  `stream` is unused (the graph runs on the

@@ -41,6 +41,9 @@ SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip"
            "tb_host.cpp", "tb.hip"]
 DRIVER_SRC = os.path.join(HERE, "driver", "ldpc_ber.cpp")
 DRIVER = os.path.join(HERE, "bin", "ldpc_ber")
+# host programs the GPU tests run (built here, so that no test compiles on the GPU machine)
+TEST_PROGRAMS = {os.path.join(ROOT, "tests", "compat", "c_entry.cpp"): os.path.join(ROOT, "tests", "compat", "_bin",
+                                                                                      "c_entry")}
 
 
 def _compile(src):
@@ -75,6 +78,7 @@ def build(verbose=True):
     if verbose:
         print("built", os.path.relpath(LIB, ROOT))
     build_driver(verbose)
+    build_test_programs(verbose)
     return LIB
 
 
@@ -91,6 +95,24 @@ def build_driver(verbose=True):
     if verbose:
         print("built", os.path.relpath(DRIVER, ROOT))
     return DRIVER
+
+
+def build_test_programs(verbose=True):
+    """Host programs of the GPU tests, linked against the library (rpath to lib/)."""
+    for src, exe in TEST_PROGRAMS.items():
+        if not os.path.exists(src):
+            continue
+        os.makedirs(os.path.dirname(exe), exist_ok=True)
+        tmp = exe + ".tmp"
+        rel = os.path.relpath(LIBDIR, os.path.dirname(exe))
+        cmd = [HIPCC, "-O1", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}", src, f"-L{LIBDIR}", "-lnrldpc",
+               f"-Wl,-rpath,$ORIGIN/{rel}", "-o", tmp]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"test program build failed for {src}:\n" + r.stdout + r.stderr)
+        os.replace(tmp, exe)
+        if verbose:
+            print("built", os.path.relpath(exe, ROOT))
 
 
 if __name__ == "__main__":

@@ -35,10 +35,6 @@
 #include <type_traits>
 #include <utility>
 
-#ifndef NRLDPC_LDS_ATOMIC_ACC
-#define NRLDPC_LDS_ATOMIC_ACC 0
-#endif
-
 namespace nrldpc {
 
 template <int B, int E, class F>
@@ -540,22 +536,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 return;
             }
             // VN update, ascending block row per column
-#if NRLDPC_LDS_ATOMIC_ACC
-            static_for<0, D>([&](auto KC) {
-                constexpr int k = KC;
-                constexpr int e = E0 + k;
-                constexpr int j = S::T.ecol[e];
-                if constexpr (j < NND) {
-                    const uint32_t addr = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
-                    if (!MIXED || !dead)
-                    __hip_atomic_fetch_add((lds_f32*)(uintptr_t)addr, dtv[k], __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_WORKGROUP);
-                } else {
-                    float& ap = app1[S::T.d1_slot[j]];
-                    ap = ap + dtv[k];
-                }
-            });
-#else
             // read-modify-write in chunks of 8 edges: loads first, then stores
             static_for<0, (D + 7) / 8>([&](auto CC) {
                 constexpr int C0 = CC * 8;
@@ -583,7 +563,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     }
                 });
             });
-#endif
         });
         __syncthreads();
         const bool last = t + 1 == a.n_iter;

@@ -65,8 +65,12 @@ struct ldpc_mixed {
     unsigned* d_queue = nullptr;
     int n_tiles = 0, n_cw = 0;
     std::vector<int> order, tiles6;
+    // iteration count of the learned weights in the plan's tiles: 0 none, -1 the plan mixes
+    // codes whose weights are for different iteration counts
     int plan_trained_T = 0;
     bool planned = false;
+    // recorded on the stream of each launch (decode or replay) after its last use of the plan,
+    // the work counter and the caller's buffers; the next launch of this object waits for it
     hipEvent_t uploaded = nullptr;
     bool upload_pending = false;
     hipGraph_t graph = nullptr;
@@ -77,7 +81,9 @@ struct ldpc_mixed {
     float* g_llr = nullptr;
     uint8_t* g_hd = nullptr;
     uint8_t* g_hd_host = nullptr;
-    size_t g_llr_floats = 0, g_hd_bytes = 0;
+    size_t g_llr_floats = 0, g_hd_bytes = 0;  // buffer sizes given at capture (0: not given)
+    int g_n_iter = 0;                         // iteration count and flags of the captured decode
+    unsigned g_flags = 0;
     long long plan_llr_extent = 0, plan_hd_extent = 0;  // floats / bytes the current plan touches
     std::mutex mu;
 };
@@ -126,6 +132,18 @@ static int wait_upload(ldpc_mixed* p) {
     return LDPC_OK;
 }
 
+// Launches of one object share its plan copy and its work counter, so a launch on stream s
+// waits (on the GPU) for the previous launch of the object, whatever stream that used.  A
+// stream under capture cannot wait for an event recorded outside it: such a caller orders the
+// object's launches itself.
+static int order_after_previous(ldpc_mixed* p, hipStream_t s) {
+    if (!p->upload_pending) return LDPC_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    NRLDPC_HIP(hipStreamIsCapturing(s, &cs));
+    if (cs == hipStreamCaptureStatusNone) NRLDPC_HIP(hipStreamWaitEvent(s, p->uploaded, 0));
+    return LDPC_OK;
+}
+
 static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
     if (n < 0 || n > p->max_cw || (n > 0 && !cws)) {
         set_last_error("mixed plan: codeword count outside 0..max_cw");
@@ -160,7 +178,10 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
         std::vector<int>& v = g.second;
         std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return cws[a].m_rows > cws[b].m_rows; });
         const int cpb = mixed_cpb(mc);
-        if (mc.d_weights) trained_T = mc.weights_iters;
+        if (mc.d_weights) {
+            if (trained_T == 0) trained_T = mc.weights_iters;
+            else if (trained_T != mc.weights_iters) trained_T = -1;
+        }
         for (size_t s = 0; s < v.size(); s += cpb) {
             HostTile h{};
             h.t.etab = mc.d_etab;
@@ -227,6 +248,22 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
     return LDPC_OK;
 }
 
+// The plan's LLR (and APP) floats and hard-decision bytes against the buffer sizes a caller
+// gave (0: size not given, not checked).
+static int check_plan_fits(const ldpc_mixed* p, size_t llr_floats, size_t hd_bytes) {
+    if (llr_floats && (size_t)p->plan_llr_extent > llr_floats) {
+        set_last_error("mixed: the plan addresses " + std::to_string(p->plan_llr_extent) +
+                       " LLR floats, beyond the " + std::to_string(llr_floats) + " given at capture");
+        return LDPC_EINVAL;
+    }
+    if (hd_bytes && (size_t)p->plan_hd_extent > hd_bytes) {
+        set_last_error("mixed: the plan addresses " + std::to_string(p->plan_hd_extent) +
+                       " hard-decision bytes, beyond the " + std::to_string(hd_bytes) + " given at capture");
+        return LDPC_EINVAL;
+    }
+    return LDPC_OK;
+}
+
 static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flags, uint8_t* hd, float* app_out,
                       int* iters_out, MixedArgs* out) {
     const bool learned = (flags & LDPC_FLAG_LEARNED) != 0;
@@ -235,7 +272,9 @@ static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flag
         return LDPC_EINVAL;
     }
     if (learned && p->plan_trained_T && p->plan_trained_T != n_iter) {
-        set_last_error("mixed decode: learned weights in the plan are for another iteration count");
+        set_last_error(p->plan_trained_T < 0
+                           ? "mixed decode: the plan holds learned weights for different iteration counts"
+                           : "mixed decode: learned weights in the plan are for another iteration count");
         return LDPC_EINVAL;
     }
     MixedArgs& a = *out;
@@ -375,8 +414,9 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
     NRLDPC_HIP(hipSetDevice(p->device));
     if (p->n_tiles == 0) return LDPC_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    int st = mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out);
+    int st = order_after_previous(p, s);
     if (st) return st;
+    if ((st = mixed_enqueue(p, s, llr, n_iter, flags, hd, app_out, iters_out))) return st;
     NRLDPC_HIP(hipEventRecord(p->uploaded, s));
     p->upload_pending = true;
     return LDPC_OK;
@@ -390,10 +430,15 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
     if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
     std::lock_guard<std::mutex> lk(p->mu);
     if (!p->planned) { set_last_error("mixed: plan a batch before capturing"); return LDPC_EINVAL; }
+    if ((llr_host && !llr_floats) || (hd_host && !hd_bytes)) {
+        set_last_error("mixed capture: a host buffer needs its size");
+        return LDPC_EINVAL;
+    }
     NRLDPC_HIP(hipSetDevice(p->device));
     MixedArgs a;
     int st = mixed_args(p, llr, n_iter, flags, hd, app_out, iters_out, &a);
     if (st) return st;
+    if ((st = check_plan_fits(p, llr_floats, hd_bytes))) return st;
     if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
     if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
     p->n_head = p->n_body = nullptr;
@@ -416,10 +461,12 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
     (void)kern;
     p->g_llr_host = llr_host;
     p->g_llr = const_cast<float*>(llr);
-    p->g_llr_floats = llr_host ? llr_floats : 0;
+    p->g_llr_floats = llr_floats;
     p->g_hd = hd;
     p->g_hd_host = hd_host;
-    p->g_hd_bytes = hd_host ? hd_bytes : 0;
+    p->g_hd_bytes = hd_bytes;
+    p->g_n_iter = n_iter;
+    p->g_flags = flags;
     NRLDPC_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     return LDPC_OK;
 }
@@ -429,16 +476,25 @@ int ldpc_mixed_replay(ldpc_mixed* p, void* stream) {
     std::lock_guard<std::mutex> lk(p->mu);
     if (!p->exec) { set_last_error("mixed: nothing captured"); return LDPC_EINVAL; }
     if (!p->planned) { set_last_error("mixed: no plan"); return LDPC_EINVAL; }
+    // The current plan must fit what was captured: the buffers' sizes and, for learned tiles,
+    // the weights' iteration count (the graph's kernel reads weights of n_iter iterations).
+    int st = check_plan_fits(p, p->g_llr_floats, p->g_hd_bytes);
+    if (st) return st;
+    if ((p->g_flags & LDPC_FLAG_LEARNED) && p->plan_trained_T && p->plan_trained_T != p->g_n_iter) {
+        set_last_error("mixed replay: the plan's learned weights are not for the captured iteration count");
+        return LDPC_EINVAL;
+    }
     NRLDPC_HIP(hipSetDevice(p->device));
-    // The previous replay has finished (planning waited for it), so the copies can be resized
-    // to what the current plan reads and writes.
-    const size_t lf = (size_t)std::min<long long>((long long)p->g_llr_floats, p->plan_llr_extent);
-    const size_t hb = (size_t)std::min<long long>((long long)p->g_hd_bytes, p->plan_hd_extent);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if ((st = order_after_previous(p, s))) return st;
+    // The previous launch of this object has finished using the plan (planning waited for it),
+    // so the copies can be resized to what the current plan reads and writes.
+    const size_t lf = p->g_llr_host ? (size_t)p->plan_llr_extent : 0;
+    const size_t hb = p->g_hd_host ? (size_t)p->plan_hd_extent : 0;
     NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_head, p->d_plan, p->h_plan, plan_head_bytes(p),
                                                  hipMemcpyHostToDevice));
     NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_body, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
                                                  plan_body_bytes(p), hipMemcpyHostToDevice));
-    hipStream_t s = static_cast<hipStream_t>(stream);
     if (lf) NRLDPC_HIP(hipMemcpyAsync(p->g_llr, p->g_llr_host, lf * sizeof(float), hipMemcpyHostToDevice, s));
     NRLDPC_HIP(hipGraphLaunch(p->exec, s));
     if (hb) NRLDPC_HIP(hipMemcpyAsync(p->g_hd_host, p->g_hd, hb, hipMemcpyDeviceToHost, s));
@@ -541,6 +597,7 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
         mc->cpb = cand[c];
         if ((st = mixed_plan_locked(p, d.data(), n_cw))) break;
         float ms = 0.f;
+        if ((st = order_after_previous(p, nullptr))) break;
         for (int rep = 0; rep < 4 && st == LDPC_OK; rep++) {
             if (rep == 1) (void)hipEventRecord(e0, nullptr);
             st = mixed_enqueue(p, nullptr, llr, n_iter, 0, hd, nullptr, nullptr);

@@ -81,6 +81,9 @@ int ldpc_noms_set_weights(ldpc_ctx* x, int n_iter, const float* w) {
     std::vector<float> h((size_t)(n_iter + 1) * nnz * 2, 0.f);
     std::copy(w, w + (size_t)n_iter * nnz * 2, h.begin() + (size_t)nnz * 2);
     std::lock_guard<std::mutex> lk(x->mu);
+    // Learned decodes, fallbacks and Adam updates on any stream may still read or write the
+    // table: replace it only once the device is idle.
+    NRLDPC_HIP(hipDeviceSynchronize());
     if (x->d_weights && x->weights_iters != n_iter) {
         NRLDPC_HIP(hipFree(x->d_weights));
         x->d_weights = nullptr;

@@ -428,9 +428,10 @@ long long oracle_decode_char(const oracle_code *c, const int8_t *llr, int n_cw, 
     return oor;
 }
 
-void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter,
-                        const float *alpha, const float *beta, uint8_t *hd, float *app) {
-    decode_many(c, llr, n_cw, n_iter, 0, alpha, beta, hd, app, NULL, 1, 0);
+void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
+                        const float *alpha, const float *beta, uint8_t *hd, float *app, int *iters,
+                        int threads) {
+    decode_many(c, llr, n_cw, n_iter, early_exit, alpha, beta, hd, app, iters, threads, PREC_FLOAT);
 }
 
 /* ---------------------------------------------------------------- NOMS in double + gradient */

@@ -62,10 +62,14 @@ uint16_t oracle_float_to_half(float f);  /* round to nearest even */
 float oracle_half_to_float(uint16_t h);
 
 /* Learned offset/normalized min-sum: R = alpha[t][e] * sign * max(m - beta[t][e], 0) with one
- * (alpha, beta) per (iteration, base-graph edge); alpha = 0.75, beta = 0 is the plain decoder
- * above.  Single precision, same schedule as oracle_decode. */
-void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter,
-                        const float *alpha, const float *beta, uint8_t *hd, float *app);
+ * (alpha, beta) per (iteration, base-graph edge in use, [n_iter][nnz]); alpha = 0.75, beta = 0
+ * is the plain decoder above.  Single precision, same schedule as oracle_decode (the
+ * reference's fixed-iteration loop, src/gpu_decoder_top.cu:25-47, with the constant 0.75 of
+ * src/ldpc_kernels.cu:164,295 replaced by the weights), including early_exit, iters and
+ * threads as there. */
+void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_iter, int early_exit,
+                        const float *alpha, const float *beta, uint8_t *hd, float *app, int *iters,
+                        int threads);
 
 /* Double-precision NOMS forward + mean BCE loss over all cols*Z bits (target bit x, soft
  * output p = sigmoid(-APP)), and its analytic gradient w.r.t. alpha and beta

@@ -28,7 +28,8 @@ class Oracle:
         lib.oracle_code_free.argtypes = [ctypes.POINTER(OracleCode)]
         lib.oracle_decode.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       P, P, P, ctypes.c_int]
-        lib.oracle_decode_noms.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, P, P, P, P]
+        lib.oracle_decode_noms.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           P, P, P, P, P, ctypes.c_int]
         lib.oracle_decode_half.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            P, P, P, ctypes.c_int]
         lib.oracle_decode_char.argtypes = [ctypes.POINTER(OracleCode), P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -98,16 +99,20 @@ class Oracle:
                                           hd.ctypes.data, app.ctypes.data, iters.ctypes.data, threads)
         return hd, app, iters, int(oor)
 
-    def decode_noms(self, c, llr, n_iter, alpha, beta):
+    def decode_noms(self, c, llr, n_iter, alpha, beta, early_exit=False, threads=8, with_iters=False):
+        """Learned decode; alpha, beta: [n_iter][nnz of the rows in use] (flattened).  Returns
+        hd, app (and iters when with_iters)."""
         llr = np.ascontiguousarray(llr, dtype=np.float32)
         n_cw, nvn = llr.shape
         hd = np.empty((n_cw, nvn), dtype=np.uint8)
         app = np.empty((n_cw, nvn), dtype=np.float32)
+        iters = np.empty(n_cw, dtype=np.int32)
         alpha = np.ascontiguousarray(alpha, dtype=np.float32)
         beta = np.ascontiguousarray(beta, dtype=np.float32)
-        self.lib.oracle_decode_noms(ctypes.byref(c), llr.ctypes.data, n_cw, n_iter, alpha.ctypes.data,
-                                    beta.ctypes.data, hd.ctypes.data, app.ctypes.data)
-        return hd, app
+        self.lib.oracle_decode_noms(ctypes.byref(c), llr.ctypes.data, n_cw, n_iter, int(early_exit),
+                                    alpha.ctypes.data, beta.ctypes.data, hd.ctypes.data, app.ctypes.data,
+                                    iters.ctypes.data, threads)
+        return (hd, app, iters) if with_iters else (hd, app)
 
     def noms_loss_grad(self, c, llr, bits, n_iter, alpha, beta, grad=True):
         llr = np.ascontiguousarray(llr, dtype=np.float32)

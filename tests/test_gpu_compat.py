@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.oracle_binding import OR_BG1, OR_WIFI
+from tests.oracle_binding import OR_BG1, OR_BG2, OR_WIFI
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -90,16 +90,34 @@ def test_stream_packing_for_odd_lifting_size(nr, torch_gpu):
     assert np.array_equal(d_out.cpu().numpy(), np.packbits(bits.reshape(-1), bitorder="little"))
 
 
-def test_c_entry_point_matches_c_abi(tmp_path):
-    """ldpc_gpu_decoder(char*, char*) on int8 LLRs equals the C ABI's CHAR_PRC decode of them."""
-    print("case C entry point BG1 Z=384", flush=True)
-    exe = str(tmp_path / "c_entry")
-    lib = os.path.join(ROOT, "nr_gpu_ldpc_amd", "lib")
-    subprocess.run(["/opt/rocm/bin/hipcc", "-O1", "-std=c++17", f"-I{ROOT}/include",
-                    f"{ROOT}/tests/compat/c_entry.cpp", f"-L{lib}", "-lnrldpc", f"-Wl,-rpath,{lib}", "-o", exe],
-                   check=True, capture_output=True, timeout=120)
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=100)
+C_ENTRY = os.path.join(ROOT, "tests", "compat", "_bin", "c_entry")
+
+
+@pytest.mark.parametrize("A,rate,cpm,n_mcw,ebn0,bg,Z", [
+    (8424, 1 / 3, 2, 4, 1.5, 1, 384),   # the reference's shipped main(): N_before_puncture % 8 == 0
+    (62, 0.2, 3, 5, 2.0, 2, 13),        # odd Z: 52 * 13 = 676 columns, the stream-repack leg
+])
+def test_c_entry_point_matches_oracle(oracle, tmp_path, A, rate, cpm, n_mcw, ebn0, bg, Z):
+    """ldpc_gpu_decoder(char*, char*) (Linking_With_Other_Projects.md:19-25) on the int8 LLRs that
+    quantize(0.5, x, 8) makes: its packed output stream equals the oracle's CHAR_PRC decode of
+    the same inputs, packed over the concatenated [n_cw][N_before_puncture] columns as
+    pack_hard_decision lays them out; and it equals the C ABI's decode + stream repack."""
+    print(f"case C entry point A={A} BG{bg} Z={Z}", flush=True)
+    prefix = str(tmp_path / "centry")
+    n_iter = 10
+    r = subprocess.run([C_ENTRY, str(A), str(rate), str(cpm), str(n_mcw), str(n_iter), str(ebn0), prefix],
+                       capture_output=True, text=True, timeout=100)
     assert r.returncode == 0, r.stderr
     out = json.loads(r.stdout)
-    assert out["status"] == 0 and out["differing_bytes"] == 0 and out["bytes"] == (8 * 26112 + 7) // 8
-    assert out["ones"] > 0
+    n_cw = cpm * n_mcw
+    n_vn = out["n_vn"]
+    assert (out["bg"], out["Z"], out["n_cw"]) == (bg, Z, n_cw)
+    assert n_vn == (68 if bg == 1 else 52) * Z
+    assert out["status"] == 0 and out["differing_bytes"] == 0 and out["bytes"] == (n_cw * n_vn + 7) // 8
+    q = np.fromfile(prefix + ".llr", dtype=np.int8).reshape(n_cw, n_vn)
+    got = np.fromfile(prefix + ".out", dtype=np.uint8)
+    ohd, _, _, oor = oracle.decode_char(oracle.code(OR_BG1 if bg == 1 else OR_BG2, Z), q, n_iter)
+    want = np.packbits(ohd.reshape(-1), bitorder="little")
+    assert np.array_equal(got, want)
+    assert 0 < int(np.unpackbits(got).sum()) < n_cw * n_vn  # neither all zeros nor all ones
+    assert oor == 0

@@ -235,6 +235,70 @@ def test_oracle_noms_matches_dense_restatement(oracle, kind, Z, m):
     assert not np.array_equal(papp, app)
 
 
+@pytest.mark.parametrize("kind,Z", [(OR_BG2, 6), (OR_BG1, 5), (OR_WIFI, 7)])
+def test_noms_early_exit_plain_weights_equal_plain_early_exit(oracle, kind, Z):
+    """alpha = 0.75, beta = 0 with early exit is the plain decoder with early exit, bit for bit,
+    iteration counts included (on codewords that stop at different iterations)."""
+    code = oracle.code(kind, Z)
+    nnz = int((oracle.shifts(code) >= 0).sum())
+    rng = np.random.default_rng(11 * Z + kind)
+    n = code.cols * Z
+    # all-zero codeword (valid for every code) through noise of several strengths
+    llr = (2.0 + rng.standard_normal((24, n)) * np.linspace(0.4, 4.0, 24)[:, None]).astype(F)
+    if kind != OR_WIFI:
+        llr[:, :2 * Z] = 0.0
+    T = 12
+    a = np.full(T * nnz, 0.75, dtype=F)
+    b = np.zeros(T * nnz, dtype=F)
+    hd, app, it = oracle.decode_noms(code, llr, T, a, b, early_exit=True, with_iters=True)
+    phd, papp, pit = oracle.decode(code, llr, T, early_exit=True)
+    assert np.array_equal(it, pit) and np.array_equal(hd, phd)
+    assert np.array_equal(app.view(np.uint32), papp.view(np.uint32))
+    assert len(set(it.tolist())) >= 3 and it.min() < T and it.max() == T
+
+
+@pytest.mark.parametrize("kind,Z,m", [(OR_BG2, 3, 0), (OR_WIFI, 4, 0), (OR_BG1, 2, 8)])
+def test_noms_early_exit_is_first_zero_syndrome_of_fixed_schedule(oracle, kind, Z, m):
+    """Learned decode with early exit and general weights = the dense restatement run for T'
+    iterations, where T' is the first iteration count whose hard decision has zero syndrome by
+    brute force (H x over GF(2)), or T if none: hard decisions, APP and iteration counts."""
+    code = oracle.code(kind, Z, m)
+    H = _dense_H(oracle, code)
+    V = oracle.shifts(code)
+    eidx = np.full(H.shape, -1, dtype=np.int64)
+    e = 0
+    for i in range(V.shape[0]):
+        for j in range(V.shape[1]):
+            if V[i, j] >= 0:
+                for r in range(Z):
+                    eidx[i * Z + r, j * Z + (r + V[i, j]) % Z] = e
+                e += 1
+    T = 6
+    rng = np.random.default_rng(5 * Z + kind + m)
+    alpha = rng.uniform(0.55, 1.0, (T, e)).astype(F)
+    beta = rng.uniform(0.0, 0.4, (T, e)).astype(F)
+    n = H.shape[1]
+    llr = (1.5 + rng.standard_normal((6, n)) * np.linspace(0.5, 2.0, 6)[:, None]).astype(F)
+    if kind != OR_WIFI:
+        llr[:, :2 * Z] = 0
+    hd, app, it = oracle.decode_noms(code, llr, T, alpha.reshape(-1), beta.reshape(-1), early_exit=True,
+                                     with_iters=True)
+    seen = set()
+    for w in range(llr.shape[0]):
+        stop = T
+        for t in range(1, T + 1):
+            dhd, dapp = dense_minsum(H, llr[w], t, alpha=alpha[:t], beta=beta[:t], eidx=eidx)
+            if not (H.astype(np.int64) @ dhd.astype(np.int64) % 2).any():
+                stop = t
+                break
+        dhd, dapp = dense_minsum(H, llr[w], stop, alpha=alpha[:stop], beta=beta[:stop], eidx=eidx)
+        assert it[w] == stop
+        assert np.array_equal(hd[w], dhd)
+        assert np.array_equal(app[w].view(np.uint32), dapp.view(np.uint32))
+        seen.add(stop)
+    assert len(seen) >= 2
+
+
 def test_noms_gradient_matches_finite_differences(oracle, nr):
     code = oracle.code(OR_BG2, 3)
     nnz = int((oracle.shifts(code) >= 0).sum())

@@ -134,6 +134,59 @@ def test_setup_params_closed_forms(nr):
     assert (p["n_nz_in_row"], p["n_nz_in_col"]) == (10, 23)
 
 
+# Known answers worked by hand from TS 38.212 (5.1 CRC length, 7.2.2 base-graph choice, 5.2.2
+# K_b, Table 5.3.2-1 lifting sizes) with the reference's own two deviations kept
+# (src/nr_ldpc.cpp:37-112): Z_min = floor(K'/K_b) where the standard asks for K_b * Z >= K', and
+# the rate compared as a float promoted to double.  Tuple: A, rate -> bg, Z, i_LS, K.
+SETUP_KNOWN = [
+    # BG choice: A <= 292 takes BG2 at any rate; B = 308 -> K_b = 8, Z >= 38 -> 40 (5 * 8)
+    (292, 0.9, 2, 40, 2, 400),
+    # A = 293 at a high rate: BG1; B = 309, Z >= floor(309/22) = 14 -> 14 (7 * 2), so K = 308 < B
+    # (the reference's floor; the standard would take Z = 15)
+    (293, 0.9, 1, 14, 3, 308),
+    # A < 3824 and R <= 0.67: BG2 (B = 3016, K_b = 10, Z >= 301 -> 320 = 5 * 64) ...
+    (3000, 0.66, 2, 320, 2, 3200),
+    # ... 0.67 as a float is 0.67000001... > 0.67: BG1 (Z >= 137 -> 144 = 9 * 16)
+    (3000, 0.67, 1, 144, 4, 3168),
+    (3000, 0.7, 1, 144, 4, 3168),
+    # A = 3823 < 3824: 16 CRC bits, BG2, B = 3839 <= K_cb = 3840, Z >= 383 -> 384
+    (3823, 0.5, 2, 384, 1, 3840),
+    # A = 3824: still 16 CRC bits but not < 3824, so BG1 at R = 0.5; B = 3840, Z >= 174 -> 176
+    (3824, 0.5, 1, 176, 5, 3872),
+    # A = 3825: 24 CRC bits (A > 3824), B = 3849, BG1, Z >= 174 -> 176
+    (3825, 0.5, 1, 176, 5, 3872),
+    # R <= 0.25 takes BG2 where neither A rule does (A = 3824: B = 3840 = K_cb, Z >= 384 -> 384)
+    (3824, 0.25, 2, 384, 1, 3840),
+    # BG2 K_b thresholds of 5.2.2: B = 192 -> 6; 193 -> 8; 560 -> 8; 561 -> 9; 640 -> 9; 641 -> 10
+    (176, 0.2, 2, 32, 0, 320),    # Z >= 192/6 = 32 -> 32 (2 * 16)
+    (177, 0.2, 2, 24, 1, 240),    # Z >= floor(193/8) = 24 -> 24 (3 * 8)
+    (544, 0.2, 2, 72, 4, 720),    # Z >= 560/8 = 70 -> 72 (9 * 8)
+    (545, 0.2, 2, 64, 0, 640),    # Z >= floor(561/9) = 62 -> 64 (2 * 32)
+    (624, 0.2, 2, 72, 4, 720),    # Z >= floor(640/9) = 71 -> 72
+    (625, 0.2, 2, 64, 0, 640),    # Z >= floor(641/10) = 64 -> 64
+]
+
+
+@pytest.mark.parametrize("A,rate,bg,Z,ils,K", SETUP_KNOWN)
+def test_setup_params_known_answers(nr, oracle, A, rate, bg, Z, ils, K):
+    p = nr.setup_params(A, rate)
+    assert (p["bg"], p["Z"], p["ils"], p["K"]) == (bg, Z, ils, K)
+    assert p["N"] == (66 if bg == 1 else 50) * Z and p["N_before_puncture"] == p["N"] + 2 * Z
+    assert (p["H_rows"], p["H_cols"]) == ((46, 68) if bg == 1 else (42, 52))
+    assert list(p.values()) == oracle.setup_ldpc(A, rate)
+
+
+@pytest.mark.parametrize("A,rate", [(8425, 1 / 3), (3841, 0.2), (5000, 0.25), (20000, 0.5)])
+def test_setup_params_refuses_segmentation(nr, A, rate):
+    """B = A + L beyond the base graph's K_cb (8448 for BG1, 3840 for BG2) needs C > 1 code
+    blocks (5.2.2), which the reference's setup_ldpc does not support (src/nr_ldpc.cpp:78-81):
+    the library says so instead of returning an invalid lifting size; the transport-block chain
+    (ldpc_tb_create) segments such blocks."""
+    with pytest.raises(nr.LdpcError) as e:
+        nr.setup_params(A, rate)
+    assert e.value.status == -5
+
+
 def test_invalid_codes_rejected(nr):
     with pytest.raises(nr.LdpcError):
         nr.Code(1, 17, 0, -1)        # not a lifting size
