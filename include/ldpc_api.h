@@ -182,6 +182,25 @@ int ldpc_noms_get_weights(ldpc_ctx* ctx, int* n_iter, float* weights);
 int ldpc_decode_learned_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
                               unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 
+/* Size of the fixed device array that ldpc_decode_learned_noalloc_async uses for its
+ * temporaries: one array per GPU, held from the library's first use of that GPU until the
+ * process ends. */
+#define LDPC_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* ldpc_decode_learned_async that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, outputs and results as
+ * ldpc_decode_learned_async; with LDPC_FLAG_FALLBACK_PLAIN the fallback's temporaries live in the
+ * fixed per-GPU array (LDPC_NOALLOC_SCRATCH_BYTES) instead of the context's grow-only scratch, and
+ * the call makes no hipMalloc, hipFree or event.  Returns LDPC_ENOMEM, enqueueing nothing, when
+ * the batch needs more than that array (per codeword about 16 bytes + hd_bytes, plus n_vn floats
+ * with app_out: at BG1 Z=384 up to about 10,200 codewords without app_out, 310 with it).  Every
+ * call on one GPU shares that GPU's array, whatever context or stream issues it, and finds it
+ * holding the previous call's contents (nothing in it needs clearing): calls that overlap on one
+ * GPU race and give wrong results, so the caller orders them. */
+int ldpc_decode_learned_noalloc_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, int n_iter,
+                                      unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
+
 /* Workspace bytes for ldpc_noms_train_grad_async with n_cw codewords per call. */
 size_t ldpc_noms_workspace_bytes(const ldpc_ctx* ctx, int n_cw, int n_iter);
 

@@ -22,6 +22,7 @@ BG1, BG2, QC12X24 = 1, 2, 3
 EARLY_EXIT = 1
 FALLBACK_PLAIN = 2
 LEARNED = 4
+NOALLOC_SCRATCH_BYTES = 32 << 20  # LDPC_NOALLOC_SCRATCH_BYTES of ldpc_api.h
 CRC16, CRC24A, CRC24B = 0, 1, 2
 
 if not os.path.exists(LIB_PATH):
@@ -95,6 +96,7 @@ _sig("ldpc_count_errors_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_noms_set_weights", ctypes.c_int, _P, ctypes.c_int, _P)
 _sig("ldpc_noms_get_weights", ctypes.c_int, _P, _c_int_p, _P)
 _sig("ldpc_decode_learned_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
+_sig("ldpc_decode_learned_noalloc_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_noms_workspace_bytes", ctypes.c_size_t, _P, ctypes.c_int, ctypes.c_int)
 _sig("ldpc_noms_train_grad_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P,
      ctypes.c_size_t)
@@ -271,6 +273,13 @@ class Code:
     def decode_learned(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
         _check(lib.ldpc_decode_learned_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,
                                              app_ptr or None, iters_ptr or None), "ldpc_decode_learned_async")
+
+    def decode_learned_noalloc(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
+        """decode_learned with the fallback's temporaries in the library's fixed per-GPU array
+        (nothing is allocated; at most NOALLOC_SCRATCH_BYTES of temporaries)."""
+        _check(lib.ldpc_decode_learned_noalloc_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,
+                                                      app_ptr or None, iters_ptr or None),
+               "ldpc_decode_learned_noalloc_async")
 
     def workspace_bytes(self, n_cw, n_iter):
         return lib.ldpc_noms_workspace_bytes(self._h, n_cw, n_iter)

@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "../../include/ldpc_api.h"
@@ -67,6 +69,86 @@ static size_t records_per_cw(const ldpc_ctx* x, int n_iter) {
     return (size_t)n_iter * rows_full * x->code.Z;
 }
 
+// Layout of the learned fallback's scratch for n_cw codewords (byte offsets, 256-aligned).
+struct FallbackLayout {
+    size_t w2, it, list, cnt, hd, app, total;
+};
+static FallbackLayout fallback_layout(const CodeDesc& c, int n_cw, bool with_app) {
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    FallbackLayout l;
+    l.w2 = up((size_t)n_cw * sizeof(int));
+    l.it = l.w2 + up((size_t)n_cw * sizeof(int));
+    l.list = l.it + up((size_t)n_cw * sizeof(int));
+    l.cnt = l.list + up((size_t)n_cw * sizeof(int));
+    l.hd = l.cnt + 256;
+    l.app = l.hd + up((size_t)n_cw * c.hd_bytes());
+    l.total = l.app + (with_app ? (size_t)n_cw * c.n_vn() * sizeof(float) : 0);
+    return l;
+}
+
+// Learned decode with plain min-sum fallback, its temporaries in `sp` (device, l.total bytes).
+// Plain min-sum re-decodes the codewords whose learned hard decision fails the syndrome check;
+// its outputs replace the learned ones only where the plain hard decision passes the check (a
+// learned output with correct information bits but an unsatisfied parity check is kept when
+// plain min-sum does no better).  Every scratch word the kernels read is written earlier in the
+// same call (failure flags of every codeword, the compacted list and its count, the re-decode's
+// outputs of every listed codeword), so the scratch needs no clearing.
+static int decode_fallback(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
+                           uint8_t* hd, float* app_out, int* iters_out, const FallbackLayout& l, char* sp) {
+    const CodeDesc& c = x->code;
+    int* w1 = reinterpret_cast<int*>(sp);
+    int* w2 = reinterpret_cast<int*>(sp + l.w2);
+    int* it2 = reinterpret_cast<int*>(sp + l.it);
+    int* list = reinterpret_cast<int*>(sp + l.list);
+    int* cnt = reinterpret_cast<int*>(sp + l.cnt);
+    uint8_t* hd2 = reinterpret_cast<uint8_t*>(sp + l.hd);
+    float* app2 = app_out ? reinterpret_cast<float*>(sp + l.app) : nullptr;
+    const hipStream_t hs = static_cast<hipStream_t>(stream);
+    int st;
+    // both decodes check the parity of their output after the last iteration (w1, w2: 1 = fails)
+    if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                          x->d_weights, nullptr, nullptr, w1)))
+        return st;
+    // the failing codewords, compacted, so that the re-decode's workgroups are all busy ones
+    if ((st = compact_failures_launch(w1, n_cw, list, cnt, hs))) return st;
+    if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd2, app2, it2, nullptr,
+                          list, cnt, w2)))
+        return st;
+    MergeArgs m{w1, w2, hd2, app2, it2, hd, app_out, iters_out, n_cw, c.hd_bytes(), c.n_vn()};
+    return fallback_merge_launch(m, hs);
+}
+
+// ---- fixed scratch of ldpc_decode_learned_noalloc_async
+// One array per GPU, loaded with the library's code object and held until the process ends.  In
+// host code `scratch` only names the symbol: its device address on each GPU comes from
+// hipGetSymbolAddress (called once per device, with that device current) and is what the kernels
+// are given.
+static __device__ __align__(256) unsigned char scratch[LDPC_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (which must be the current device).
+static int noalloc_scratch(int device, char** out) {
+    if (device < 0 || device >= kMaxDevices) {
+        set_last_error("device index beyond the fixed-scratch table");
+        return LDPC_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_last_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return LDPC_EHIP;
+        }
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = static_cast<char*>(g_scratch_ptr[device]);
+    return LDPC_OK;
+}
+
 }  // namespace nrldpc
 
 using namespace nrldpc;
@@ -119,43 +201,39 @@ int ldpc_decode_learned_async(ldpc_ctx* x, void* stream, const float* llr, int n
     if (!fb)
         return decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
                            x->d_weights, nullptr, nullptr);
-    // Fallback: plain min-sum re-decodes the codewords whose learned hard decision fails the
-    // syndrome check, into scratch; its outputs replace the learned ones only where the plain
-    // hard decision passes the check (a learned output with correct information bits but an
-    // unsatisfied parity check is kept when plain min-sum does no better).
-    const CodeDesc& c = x->code;
-    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t o_w2 = up((size_t)n_cw * sizeof(int));
-    const size_t o_it = o_w2 + up((size_t)n_cw * sizeof(int));
-    const size_t o_list = o_it + up((size_t)n_cw * sizeof(int));
-    const size_t o_cnt = o_list + up((size_t)n_cw * sizeof(int));
-    const size_t o_hd = o_cnt + 256;
-    const size_t o_app = o_hd + up((size_t)n_cw * c.hd_bytes());
-    const size_t total = o_app + (app_out ? (size_t)n_cw * c.n_vn() * sizeof(float) : 0);
-    void* scratch = nullptr;
+    const FallbackLayout l = fallback_layout(x->code, n_cw, app_out != nullptr);
+    void* scratch_mem = nullptr;
     ScratchUse use(x, static_cast<hipStream_t>(stream));
-    int st = use.acquire(total, &scratch);
+    int st = use.acquire(l.total, &scratch_mem);
     if (st) return st;
-    char* sp = static_cast<char*>(scratch);
-    int* w1 = reinterpret_cast<int*>(sp);
-    int* w2 = reinterpret_cast<int*>(sp + o_w2);
-    int* it2 = reinterpret_cast<int*>(sp + o_it);
-    int* list = reinterpret_cast<int*>(sp + o_list);
-    int* cnt = reinterpret_cast<int*>(sp + o_cnt);
-    uint8_t* hd2 = reinterpret_cast<uint8_t*>(sp + o_hd);
-    float* app2 = app_out ? reinterpret_cast<float*>(sp + o_app) : nullptr;
-    const hipStream_t hs = static_cast<hipStream_t>(stream);
-    // both decodes check the parity of their output after the last iteration (w1, w2: 1 = fails)
-    if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
-                          x->d_weights, nullptr, nullptr, w1)))
-        return st;
-    // the failing codewords, compacted, so that the re-decode's workgroups are all busy ones
-    if ((st = compact_failures_launch(w1, n_cw, list, cnt, hs))) return st;
-    if ((st = decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd2, app2, it2, nullptr,
-                          list, cnt, w2)))
-        return st;
-    MergeArgs m{w1, w2, hd2, app2, it2, hd, app_out, iters_out, n_cw, c.hd_bytes(), c.n_vn()};
-    return fallback_merge_launch(m, hs);
+    return decode_fallback(x, stream, llr, n_cw, n_iter, flags, hd, app_out, iters_out, l,
+                           static_cast<char*>(scratch_mem));
+}
+
+int ldpc_decode_learned_noalloc_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter,
+                                      unsigned flags, uint8_t* hd, float* app_out, int* iters_out) {
+    if (!x || !llr || !hd || n_cw < 0) return LDPC_EINVAL;
+    if (!x->d_weights || n_iter != x->weights_iters) {
+        set_last_error("learned weights missing or for a different iteration count");
+        return LDPC_EINVAL;
+    }
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    if (n_cw == 0) return LDPC_OK;
+    if (!(flags & LDPC_FLAG_FALLBACK_PLAIN))  // no temporaries at all
+        return decode_with(x, stream, llr, n_cw, n_iter, flags & LDPC_FLAG_EARLY_EXIT, hd, app_out, iters_out,
+                           x->d_weights, nullptr, nullptr);
+    const FallbackLayout l = fallback_layout(x->code, n_cw, app_out != nullptr);
+    if (l.total > (size_t)LDPC_NOALLOC_SCRATCH_BYTES) {
+        set_last_error("allocation-free learned decode: " + std::to_string(n_cw) + " codewords need " +
+                       std::to_string(l.total) + " bytes of scratch, beyond the fixed " +
+                       std::to_string((size_t)LDPC_NOALLOC_SCRATCH_BYTES));
+        return LDPC_ENOMEM;
+    }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    char* sp = nullptr;
+    int st = noalloc_scratch(x->device, &sp);
+    if (st) return st;
+    return decode_fallback(x, stream, llr, n_cw, n_iter, flags, hd, app_out, iters_out, l, sp);
 }
 
 size_t ldpc_noms_workspace_bytes(const ldpc_ctx* x, int n_cw, int n_iter) {
