@@ -1,8 +1,11 @@
-"""The committed learned weights (BG2 Z=208, T=10) beat plain min-sum on the GPU.
+"""The committed learned weights (BG2 Z=208, T=10) gain 0.2 dB over plain min-sum on the GPU.
 
 Frame error rates come from fresh Philox codewords decoded on the GPU.  The full evaluation
-(400k codewords per point, FER 1e-3 crossing 1.745 dB learned against 1.945 dB plain) is
-stored beside the weights; this test repeats the decisive comparison at 120k codewords.
+(400k codewords per point, FER 1e-3 crossing 1.733 dB learned against 1.945 dB plain) is stored
+beside the weights.  Here the claim itself is pinned: the learned decoder's FER at x dB is at most
+plain min-sum's FER at x + 0.2 dB, at two points on the waterfall (x = 1.625 dB with 400k codewords
+per decoder, x = 1.75 dB, next to the FER 1e-3 crossing, with 1M), with the fallback never worse
+than the learned decoder alone.
 """
 import os
 
@@ -18,14 +21,15 @@ def test_learned_gain_bg2_z208(nr, torch_gpu):
     rec = noms.load_weights(path)
     code = nr.Code(2, 208)
     code.set_weights(rec["weights"])
-    n = 120_000
     print("case learned gain BG2 Z=208 T=10", flush=True)
-    plain = noms.fer_curve(code, 10, [1.75, 1.875], n)
-    learned = noms.fer_curve(code, 10, [1.75], n, learned=True)
-    fallback = noms.fer_curve(code, 10, [1.75], n, learned=True, flags=nr.FALLBACK_PLAIN)
-    print(plain, learned, fallback, flush=True)
-    # expected about 1.5e-2 / 2.8e-3 (plain at 1.75 / 1.875 dB) against 9e-4 (learned at 1.75 dB)
-    assert learned[0]["fer"] * 5 < plain[0]["fer"]
-    assert learned[0]["fer"] * 1.5 < plain[1]["fer"]
-    assert fallback[0]["fer"] <= learned[0]["fer"]
-    assert learned[0]["syndrome_fail"] < plain[0]["syndrome_fail"]
+    for x, n in ((1.625, 400_000), (1.75, 1_000_000)):
+        plain = noms.fer_curve(code, 10, [x + 0.2], n, seed=199)[0]
+        learned = noms.fer_curve(code, 10, [x], n, learned=True, seed=299)[0]
+        print(x, plain, learned, flush=True)
+        # expected about 5.5e-3 against 4.4e-3 (x = 1.625) and 9.3e-4 against 7.9e-4 (x = 1.75)
+        assert learned["frame_errors"] <= plain["frame_errors"], (x, plain, learned)
+        assert learned["frame_errors"] > 300  # a count with a few percent of noise
+    fallback = noms.fer_curve(code, 10, [1.75], 200_000, learned=True, flags=nr.FALLBACK_PLAIN, seed=5)[0]
+    alone = noms.fer_curve(code, 10, [1.75], 200_000, learned=True, seed=5)[0]
+    assert fallback["fer"] <= alone["fer"]
+    assert alone["syndrome_fail"] < noms.fer_curve(code, 10, [1.75], 200_000, seed=5)[0]["syndrome_fail"]
