@@ -36,7 +36,7 @@ COMMON = [
     f"-I{CSRC}",
 ] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
-SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
+SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "decoder_pair.hip", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
            "encoder.cpp", "noms_api.cpp", "compat.cpp", "mixed_plan.cpp", "mixed.hip",
            "tb_host.cpp", "tb.hip"]
 DRIVER_SRC = os.path.join(HERE, "driver", "ldpc_ber.cpp")
@@ -46,9 +46,19 @@ TEST_PROGRAMS = {os.path.join(ROOT, "tests", "compat", "c_entry.cpp"): os.path.j
                                                                                       "c_entry")}
 
 
+def _newest_header():
+    hs = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".h", ".inc"))]
+    hs += [os.path.join(ROOT, "include", f) for f in os.listdir(os.path.join(ROOT, "include"))]
+    return max(os.path.getmtime(h) for h in hs)
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    # NRLDPC_INCREMENTAL=1 (development only) keeps objects newer than their source and every header
+    if os.environ.get("NRLDPC_INCREMENTAL") == "1" and os.path.exists(obj) and \
+            os.path.getmtime(obj) > max(os.path.getmtime(path), _newest_header()):
+        return obj, ""
     cmd = [HIPCC] + COMMON + ["-c", path, "-o", obj]
     if src.endswith(".cpp"):
         cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", path, "-o", obj]

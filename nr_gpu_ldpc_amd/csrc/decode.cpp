@@ -77,9 +77,9 @@ int decode_char(ldpc_ctx* x, void* stream, const int8_t* llr, int n_cw, int n_it
     a.n_vn = c.n_vn();
     a.hd_bytes = c.hd_bytes();
     a.n_iter = n_iter;
-    a.cpb = x->cpb;
+    a.cpb = decoder_pair_slots(c.kind, c.Z);
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
-    a.lds_cw_bytes = (int)decoder_lds_bytes_per_cw(c.kind, c.Z);
+    a.lds_cw_bytes = (int)decoder_pair_lds_bytes(c.kind, c.Z);
     a.nnz = nnz_full(c.kind);
     return decoder_launch_char(c.kind, a, static_cast<hipStream_t>(stream));
 }

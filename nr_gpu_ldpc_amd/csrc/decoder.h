@@ -97,8 +97,12 @@ int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
 // The reference's HALF_PRC schedule (binary16 LLRs in a.llr_half).
 int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s);
-// The reference's CHAR_PRC schedule (signed-char LLRs in a.llr_i8).
+// The reference's CHAR_PRC schedule (signed-char LLRs in a.llr_i8), two codewords per lane:
+// a.cpb = codeword pairs per workgroup (decoder_pair_slots), a.lds_cw_bytes = LDS bytes per pair
+// (decoder_pair_lds_bytes).
 int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s);
+size_t decoder_pair_lds_bytes(int kind, int Z);
+int decoder_pair_slots(int kind, int Z);
 // Learned forward that also writes the training records (generic-Z kernel).
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s);
 

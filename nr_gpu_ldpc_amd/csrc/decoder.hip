@@ -17,7 +17,7 @@ int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 template <int KIND, int ZC>
-int launch_decode_char(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
+int launch_decode_char_pair(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 
 // Lifting sizes with a dedicated kernel (all offsets compile-time): the hot configurations.
 constexpr int kBG1Special = 384, kBG2Special = 208, kQCSpecial = 96;
@@ -95,8 +95,34 @@ int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s) {
                              : launch_decode_half<kWifi, 0>(a, blocks, threads, lds, s);
 }
 
+size_t decoder_pair_lds_bytes(int kind, int Z) {
+    int nnd = 0;
+    if (kind == kBG1) nnd = Structure<kBG1>::NND;
+    else if (kind == kBG2) nnd = Structure<kBG2>::NND;
+    else nnd = Structure<kWifi>::NND;
+    // read copy and accumulator, both [NND][2Z] words, and the pair's vote word
+    return ((size_t)4 * nnd * 4 * Z + 16 + 15) & ~(size_t)15;
+}
+
+int decoder_pair_slots(int kind, int Z) {
+    // fill whole wavefronts when possible, else the most lanes per wavefront, within the LDS
+    const size_t lds = decoder_pair_lds_bytes(kind, Z);
+    const size_t lim = (size_t)max_lds_bytes();
+    int best = 1;
+    double best_eff = 0.0;
+    for (int cpb = 1; cpb * Z <= kMaxBlock && cpb * lds + 16 <= lim; cpb++) {
+        const int th = cpb * Z;
+        const double eff = (double)th / (double)(((th + 63) / 64) * 64);
+        if (eff > best_eff + 1e-9) { best_eff = eff; best = cpb; }
+    }
+    return best;
+}
+
+// The CHAR_PRC schedule, two codewords per lane: a.cpb codeword pairs per workgroup, a.lds_cw_bytes
+// LDS bytes per pair.
 int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s) {
-    const int blocks = (a.n_cw + a.cpb - 1) / a.cpb;
+    const int n_pairs = (a.n_cw + 1) / 2;
+    const int blocks = (n_pairs + a.cpb - 1) / a.cpb;
     const int threads = a.cpb * a.Z;
     const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;
     if (threads > kMaxBlock || lds > (size_t)max_lds_bytes() || blocks < 1 || !a.llr_i8) {
@@ -104,13 +130,13 @@ int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s) {
         return LDPC_EINVAL;
     }
     if (kind == kBG1)
-        return a.Z == kBG1Special ? launch_decode_char<kBG1, kBG1Special>(a, blocks, threads, lds, s)
-                                  : launch_decode_char<kBG1, 0>(a, blocks, threads, lds, s);
+        return a.Z == kBG1Special ? launch_decode_char_pair<kBG1, kBG1Special>(a, blocks, threads, lds, s)
+                                  : launch_decode_char_pair<kBG1, 0>(a, blocks, threads, lds, s);
     if (kind == kBG2)
-        return a.Z == kBG2Special ? launch_decode_char<kBG2, kBG2Special>(a, blocks, threads, lds, s)
-                                  : launch_decode_char<kBG2, 0>(a, blocks, threads, lds, s);
-    return a.Z == kQCSpecial ? launch_decode_char<kWifi, kQCSpecial>(a, blocks, threads, lds, s)
-                             : launch_decode_char<kWifi, 0>(a, blocks, threads, lds, s);
+        return a.Z == kBG2Special ? launch_decode_char_pair<kBG2, kBG2Special>(a, blocks, threads, lds, s)
+                                  : launch_decode_char_pair<kBG2, 0>(a, blocks, threads, lds, s);
+    return a.Z == kQCSpecial ? launch_decode_char_pair<kWifi, kQCSpecial>(a, blocks, threads, lds, s)
+                             : launch_decode_char_pair<kWifi, 0>(a, blocks, threads, lds, s);
 }
 
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s) {

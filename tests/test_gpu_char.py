@@ -26,8 +26,11 @@ def _char_decode(torch, code, llr_q, n_iter, flags=0):
 
 # (kind, Z, codewords, iterations, Eb/N0, rate, LLR gain): a gain above 1 scales the channel LLRs
 # before quantisation so that saturated values and out-of-range conversions occur.
+# Odd codeword counts leave the last codeword of the two-per-lane kernel without a partner.
 CASES = [(1, 384, 4, 10, 1.25, 1 / 3, 1.0), (1, 384, 4, 10, 1.5, 1 / 3, 8.0), (2, 208, 6, 10, 0.5, 0.2, 1.0),
-         (3, 96, 16, 5, 2.0, 0.5, 1.0), (1, 15, 20, 6, 1.0, 1 / 3, 4.0), (2, 13, 20, 6, 1.0, 0.2, 1.0)]
+         (3, 96, 16, 5, 2.0, 0.5, 1.0), (1, 15, 20, 6, 1.0, 1 / 3, 4.0), (2, 13, 20, 6, 1.0, 0.2, 1.0),
+         (1, 384, 5, 10, 1.5, 1 / 3, 1.0), (2, 208, 7, 10, 1.0, 0.2, 8.0), (3, 96, 9, 5, 3.0, 0.5, 1.0),
+         (2, 52, 3, 8, 1.5, 0.2, 2.0), (1, 7, 1, 4, 1.0, 1 / 3, 1.0)]
 
 
 @pytest.mark.parametrize("kind,Z,n_cw,T,ebn0,rate,gain", CASES)
@@ -45,12 +48,17 @@ def test_char_matches_oracle(nr, oracle, torch_gpu, kind, Z, n_cw, T, ebn0, rate
         assert oor > 0 and (np.abs(app) >= 127).any()  # the saturating rules were exercised
 
 
-def test_char_early_exit(nr, oracle, torch_gpu):
-    print("case char early exit BG1 Z=64", flush=True)
-    code = nr.Code(1, 64)
-    llr, _ = _noisy_llr(nr, code, 40, 2.0, 1 / 3, seed=9)
+@pytest.mark.parametrize("kind,Z,n_cw,ebn0,rate", [(1, 64, 40, 2.0, 1 / 3), (1, 384, 9, 1.75, 1 / 3),
+                                                   (2, 208, 11, 1.5, 0.2), (3, 96, 17, 3.5, 0.5)])
+def test_char_early_exit(nr, oracle, torch_gpu, kind, Z, n_cw, ebn0, rate):
+    """Codewords that converge at different iterations (the two codewords of one lane included)
+    stop at the oracle's iteration with the oracle's output."""
+    print(f"case char early exit kind={kind} Z={Z} n={n_cw}", flush=True)
+    code = nr.Code(kind, Z)
+    llr, _ = _noisy_llr(nr, code, n_cw, ebn0, rate, seed=9 + Z)
     q = _quantize(llr)
     hd, app, it = _char_decode(torch_gpu, code, q, 20, flags=nr.EARLY_EXIT)
-    ohd, oapp, oit, _ = oracle.decode_char(oracle.code(OR_BG1, 64), q, 20, early_exit=True)
+    ohd, oapp, oit, _ = oracle.decode_char(oracle.code(KIND[kind], Z), q, 20, early_exit=True)
     assert np.array_equal(it, oit) and it.min() < 20
+    assert len(set(it.tolist())) > 1
     assert np.array_equal(hd, ohd) and np.array_equal(app, oapp)
