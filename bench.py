@@ -294,7 +294,8 @@ def bench_mixed(torch, nr, args):
 
 def bench_stream(torch, nr, args):
     """BG1 Z=384 / BG2 Z=208 at mixed rates, batches of 1..8192 codewords, each batch one
-    replayed graph: LLR upload, plan upload, decode, hard-decision download."""
+    replayed graph: LLR upload, plan upload, decode, hard-decision download.  The same sequence
+    with the LLRs sent as float32, as int8 (quantize(0.5, x, 8)) and as binary16."""
     from nr_gpu_ldpc_amd import mixed as mx
     from nr_gpu_ldpc_amd import noms
     T = 10
@@ -307,29 +308,35 @@ def bench_stream(torch, nr, args):
     classes = [1 << k for k in range(14)]  # 1 .. 8192
     max_f = 8192 * 68 * 384
     max_h = 8192 * ((68 * 384 + 7) // 8)
+    # element type -> (LDPC_LLR_* code, scale, torch dtype, bytes per LLR)
+    fmts = {"f32": (nr.LLR_F32, 1.0, torch.float32, 4), "i8": (nr.LLR_I8, 0.5, torch.int8, 1),
+            "f16": (nr.LLR_F16, 1.0, torch.float16, 2)}
     d_llr = torch.zeros(max_f, dtype=torch.float32, device="cuda")
     d_hd = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
-    h_llr = torch.zeros(max_f, dtype=torch.float32).pin_memory()
+    h_llr = {f: torch.zeros(max_f, dtype=v[2]).pin_memory() for f, v in fmts.items()}
     h_hd = torch.zeros(max_h, dtype=torch.uint8).pin_memory()
     stream = torch.cuda.Stream()
     s = stream.cuda_stream
     modes = {"plain": nr.EARLY_EXIT, "learned": nr.EARLY_EXIT | nr.LEARNED | nr.FALLBACK_PLAIN}
     objs = {}
-    for mode, flags in modes.items():
-        for c in classes:
-            m = nr.Mixed(c)
-            if mode == "learned":
-                for (kind, z), w in weights.items():
-                    m.set_weights(kind, z, w)
-            m.plan(mx.describe([nr.BG1] * c, [384] * c, [46] * c)["desc"])
-            m.capture(s, d_llr.data_ptr(), T, d_hd.data_ptr(), 0, 0, flags=flags, llr_host_ptr=h_llr.data_ptr(),
-                      llr_floats=c * 68 * 384, hd_host_ptr=h_hd.data_ptr(), hd_bytes=c * ((68 * 384 + 7) // 8))
-            objs[(mode, c)] = m
+    for f, (code, scale, _, nbytes) in fmts.items():
+        for mode, flags in modes.items():
+            for c in classes:
+                m = nr.Mixed(c)
+                m.set_input(code, scale)
+                if mode == "learned":
+                    for (kind, z), w in weights.items():
+                        m.set_weights(kind, z, w)
+                m.plan(mx.describe([nr.BG1] * c, [384] * c, [46] * c)["desc"])
+                m.capture(s, d_llr.data_ptr(), T, d_hd.data_ptr(), 0, 0, flags=flags,
+                          llr_host_ptr=h_llr[f].data_ptr(), llr_floats=c * 68 * 384, hd_host_ptr=h_hd.data_ptr(),
+                          hd_bytes=c * ((68 * 384 + 7) // 8))
+                objs[(f, mode, c)] = m
     ebn0_points = (1.0, 1.5, 2.0)
     batches = [(c, int(rng.integers(c // 2 + 1, c + 1)), ebn0_points[i % 3]) for i, c in enumerate(classes * 3)]
     batches += [(8192, 8192, e) for e in ebn0_points for _ in range(2)]  # FER statistics
-    stats = {mode: {"lat": [], "info": 0, "bytes": 0, "time_ms": 0.0, "fer": {e: [0, 0] for e in ebn0_points}}
-             for mode in modes}
+    stats = {f: {mode: {"lat": [], "info": 0, "bytes": 0, "time_ms": 0.0, "fer": {e: [0, 0] for e in ebn0_points}}
+                 for mode in modes} for f in fmts}
     # the bus these batches cross: best of three pinned host-to-device copies of 256 MiB
     n_cal = 64 * 1024 * 1024
     cal = []
@@ -337,7 +344,7 @@ def bench_stream(torch, nr, args):
         for _ in range(3):
             c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             c0.record(stream)
-            d_llr[:n_cal].copy_(h_llr[:n_cal], non_blocking=True)
+            d_llr[:n_cal].copy_(h_llr["f32"][:n_cal], non_blocking=True)
             c1.record(stream)
             stream.synchronize()
             cal.append(c0.elapsed_time(c1))
@@ -350,92 +357,103 @@ def bench_stream(torch, nr, args):
         r_nom = rng.uniform(0.2, 8 / 9, n)
         mr = np.where(kind == nr.BG1, mx.rows_for_rate(nr.BG1, r_nom), mx.rows_for_rate(nr.BG2, r_nom))
         drawn.append(mx.describe(kind, Zs, mr, ebn0=np.full(n, e)))
-    # every batch's LLRs in one pinned buffer, and room for every batch's decisions, for the
-    # pipelined pass below
+    # every batch's LLRs in one pinned buffer per element type, and room for every batch's
+    # decisions, for the pipelined pass below
     l_off = np.concatenate([[0], np.cumsum([b["llr_floats"] for b in drawn])])
     h_off = np.concatenate([[0], np.cumsum([b["hd_total"] for b in drawn])])
-    all_llr = torch.empty(int(l_off[-1]), dtype=torch.float32).pin_memory()
-    all_hd = {mode: torch.zeros(int(h_off[-1]), dtype=torch.uint8).pin_memory() for mode in modes}
-    want_hd = {mode: torch.zeros(int(h_off[-1]), dtype=torch.uint8) for mode in modes}
+    all_llr = {f: torch.empty(int(l_off[-1]), dtype=v[2]).pin_memory() for f, v in fmts.items()}
+    all_hd = {(f, mode): torch.zeros(int(h_off[-1]), dtype=torch.uint8).pin_memory() for f in fmts for mode in modes}
+    want_hd = {(f, mode): torch.zeros(int(h_off[-1]), dtype=torch.uint8) for f in fmts for mode in modes}
     for bi, (c, n, e) in enumerate(batches):
         b = drawn[bi]
         llr, bits = mx.make_channel(torch, b, seed=1000 + bi, contexts=contexts)
-        h_llr[:b["llr_floats"]].copy_(llr.cpu())
-        all_llr[int(l_off[bi]):int(l_off[bi + 1])].copy_(h_llr[:b["llr_floats"]])
-        for mode in modes:
-            m = objs[(mode, c)]
-            m.plan(b["desc"])
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-            m.replay(s)
-            e1.record(stream)
-            stream.synchronize()
-            ms = e0.elapsed_time(e1)
-            want_hd[mode][int(h_off[bi]):int(h_off[bi + 1])].copy_(h_hd[:b["hd_total"]])
-            hd = h_hd[:b["hd_total"]].cuda()
-            _, fe, _ = mx.info_errors(torch, b, hd, bits)
-            st = stats[mode]
-            st["lat"].append(ms)
-            st["info"] += int(b["info_bits"].sum())
-            st["bytes"] += 4 * b["llr_floats"] + b["hd_total"]
-            st["time_ms"] += ms
-            st["fer"][e][0] += fe
-            st["fer"][e][1] += n
+        x = llr.cpu().numpy()
+        host = {"f32": torch.from_numpy(x), "i8": torch.from_numpy(mx.quantize(x)),
+                "f16": torch.from_numpy(x.astype(np.float16))}
+        for f in fmts:
+            h_llr[f][:b["llr_floats"]].copy_(host[f])
+            all_llr[f][int(l_off[bi]):int(l_off[bi + 1])].copy_(host[f])
+            for mode in modes:
+                m = objs[(f, mode, c)]
+                m.plan(b["desc"])
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                m.replay(s)
+                e1.record(stream)
+                stream.synchronize()
+                ms = e0.elapsed_time(e1)
+                want_hd[(f, mode)][int(h_off[bi]):int(h_off[bi + 1])].copy_(h_hd[:b["hd_total"]])
+                hd = h_hd[:b["hd_total"]].cuda()
+                _, fe, _ = mx.info_errors(torch, b, hd, bits)
+                st = stats[f][mode]
+                st["lat"].append(ms)
+                st["info"] += int(b["info_bits"].sum())
+                st["bytes"] += fmts[f][3] * b["llr_floats"] + b["hd_total"]
+                st["time_ms"] += ms
+                st["fer"][e][0] += fe
+                st["fer"][e][1] += n
     out = {"workload": "BG1 Z=384 and BG2 Z=208 (50/50) at rates U[1/5, 8/9] by parity-row truncation, "
                        "batches of 1..8192 codewords (sizes in (2^k/2, 2^k], 3 per power of two, plus 2 x 8192 "
                        "per Eb/N0 point), Eb/N0 1.0 / 1.5 / 2.0 dB, T=10 with early exit; each batch is one "
-                       "replayed HIP graph: LLR upload, plan upload, persistent decode, hard-decision download",
+                       "replayed HIP graph: LLR upload, plan upload, persistent decode, hard-decision download; "
+                       "top-level entries send float32 LLRs, llr_int8 / llr_half the same LLRs as "
+                       "quantize(0.5, x, 8) / binary16 (converted to float32 on load, decoded in float32)",
            "learned_weights": sorted(f"{k}/{z}" for k, z in weights),
            "pcie": {"h2d_gbs": round(h2d_gbs, 2),
-                    "note": "best pinned host-to-device copy of 256 MiB; every batch moves 4 B per LLR in "
-                            "and its packed decisions out, so info Gbps <= info bits / (bytes / h2d_gbs)"}}
+                    "note": "best pinned host-to-device copy of 256 MiB; every batch moves its LLRs in (4, 1 or 2 "
+                            "B each) and its packed decisions out, so info Gbps <= info bits / (bytes / h2d_gbs)"}}
     # The same sequence pipelined over two streams: batch i + 1 is planned, uploaded and
     # decoded while batch i is decoded or downloaded.  Each stream has its own mixed object
     # (graph: plan upload + decode) and device buffers; uploads and downloads are stream copies
     # from and to the pinned buffers holding every batch.
     n_pipe = 2
     pstreams = hip_streams(torch, n_pipe)
-    for mode, flags in modes.items():
-        slots = []
-        for k in range(n_pipe):
-            m = nr.Mixed(8192)
-            if mode == "learned":
-                for (kind, z), w in weights.items():
-                    m.set_weights(kind, z, w)
-            dl = torch.zeros(max_f, dtype=torch.float32, device="cuda")
-            dh = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
-            m.plan(drawn[0]["desc"])
-            m.capture(pstreams[k].cuda_stream, dl.data_ptr(), T, dh.data_ptr(), 0, 0, flags=flags)
-            slots.append((m, dl, dh))
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for bi, b in enumerate(drawn):
-            m, dl, dh = slots[bi % n_pipe]
-            ps = pstreams[bi % n_pipe]
-            m.plan(b["desc"])  # waits until this stream's previous batch has left its buffers
-            with torch.cuda.stream(ps):
-                dl[:b["llr_floats"]].copy_(all_llr[int(l_off[bi]):int(l_off[bi + 1])], non_blocking=True)
-            m.replay(ps.cuda_stream)
-            with torch.cuda.stream(ps):
-                all_hd[mode][int(h_off[bi]):int(h_off[bi + 1])].copy_(dh[:b["hd_total"]], non_blocking=True)
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        if not torch.equal(all_hd[mode], want_hd[mode]):
-            raise RuntimeError(f"pipelined {mode} stream decisions differ from the serial pass")
-        stats[mode]["pipelined"] = {"streams": n_pipe, "info_gbps": round(stats[mode]["info"] / (wall * 1e9), 3),
-                                    "wall_ms": round(wall * 1e3, 2)}
-        for m, _, _ in slots:
-            m.close()
-    for mode, st in stats.items():
-        out[mode] = {"info_gbps": round(st["info"] / (st["time_ms"] * 1e6), 3),
-                     "pipelined": st["pipelined"],
-                     "bytes_moved": st["bytes"],
-                     "copy_bound_info_gbps": round(st["info"] * h2d_gbs / st["bytes"], 3),
-                     "latency_ms": {"p50": round(statistics.median(st["lat"]), 3),
-                                    "p99": round(pct(st["lat"], 0.99), 3),
-                                    "kind": "device time per batch incl. host copies"},
-                     "fer": {f"{e}dB": round(v[0] / max(1, v[1]), 6) for e, v in st["fer"].items()},
-                     "frames": {f"{e}dB": v[1] for e, v in st["fer"].items()}}
+    for f, (code, scale, dtype, _) in fmts.items():
+        for mode, flags in modes.items():
+            slots = []
+            for k in range(n_pipe):
+                m = nr.Mixed(8192)
+                m.set_input(code, scale)
+                if mode == "learned":
+                    for (kind, z), w in weights.items():
+                        m.set_weights(kind, z, w)
+                dl = torch.zeros(max_f, dtype=dtype, device="cuda")
+                dh = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
+                m.plan(drawn[0]["desc"])
+                m.capture(pstreams[k].cuda_stream, dl.data_ptr(), T, dh.data_ptr(), 0, 0, flags=flags)
+                slots.append((m, dl, dh))
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for bi, b in enumerate(drawn):
+                m, dl, dh = slots[bi % n_pipe]
+                ps = pstreams[bi % n_pipe]
+                m.plan(b["desc"])  # waits until this stream's previous batch has left its buffers
+                with torch.cuda.stream(ps):
+                    dl[:b["llr_floats"]].copy_(all_llr[f][int(l_off[bi]):int(l_off[bi + 1])], non_blocking=True)
+                m.replay(ps.cuda_stream)
+                with torch.cuda.stream(ps):
+                    all_hd[(f, mode)][int(h_off[bi]):int(h_off[bi + 1])].copy_(dh[:b["hd_total"]], non_blocking=True)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            if not torch.equal(all_hd[(f, mode)], want_hd[(f, mode)]):
+                raise RuntimeError(f"pipelined {f} {mode} stream decisions differ from the serial pass")
+            stats[f][mode]["pipelined"] = {"streams": n_pipe,
+                                           "info_gbps": round(stats[f][mode]["info"] / (wall * 1e9), 3),
+                                           "wall_ms": round(wall * 1e3, 2)}
+            for m, _, _ in slots:
+                m.close()
+    for f in fmts:
+        dst = out if f == "f32" else out.setdefault({"i8": "llr_int8", "f16": "llr_half"}[f], {})
+        for mode, st in stats[f].items():
+            dst[mode] = {"info_gbps": round(st["info"] / (st["time_ms"] * 1e6), 3),
+                         "pipelined": st["pipelined"],
+                         "bytes_moved": st["bytes"],
+                         "copy_bound_info_gbps": round(st["info"] * h2d_gbs / st["bytes"], 3),
+                         "latency_ms": {"p50": round(statistics.median(st["lat"]), 3),
+                                        "p99": round(pct(st["lat"], 0.99), 3),
+                                        "kind": "device time per batch incl. host copies"},
+                         "fer": {f"{e}dB": round(v[0] / max(1, v[1]), 6) for e, v in st["fer"].items()},
+                         "frames": {f"{e}dB": v[1] for e, v in st["fer"].items()}}
     return out
 
 
@@ -472,7 +490,8 @@ def bench_pipeline(torch, nr, args):
     contexts = {}
     out = {"workload": "BG1 Z=384 / BG2 Z=208 (50/50) at rates U[1/5, 8/9], Eb/N0 1.5 dB, T=10 with "
                        "early exit; per batch: plan on the host, then one replayed HIP graph (pinned "
-                       "LLR upload, plan upload, decode, hard-decision download); S streams round-robin",
+                       "LLR upload, plan upload, decode, hard-decision download); S streams round-robin, each "
+                       "stream's persistent grid 1/S of the resident workgroups of the device",
            "latency_kind": "device time per batch on its stream (start to end event), under S-way overlap"}
     for c in (16, 256, 2048):
         slots = []
@@ -499,7 +518,17 @@ def bench_pipeline(torch, nr, args):
             slots.append({"m": m, "b": b, "h_llr": h_llr, "h_hd": h_hd, "d_llr": d_llr, "d_hd": d_hd,
                           "want": h_hd.clone(), "info": int(b["info_bits"].sum())})
         row = {}
+        full_grid = slots[0]["m"].plan_info()["grid"]
         for S in (1, 2, 4):
+            # each stream's persistent grid takes its share of the device
+            for k in range(S):
+                sl = slots[k]
+                sl["m"].set_grid(full_grid // S)
+                sl["m"].plan(sl["b"]["desc"])
+                sl["m"].capture(streams[k].cuda_stream, sl["d_llr"].data_ptr(), T, sl["d_hd"].data_ptr(), 0, 0,
+                                flags=nr.EARLY_EXIT, llr_host_ptr=sl["h_llr"].data_ptr(),
+                                llr_floats=sl["b"]["llr_floats"], hd_host_ptr=sl["h_hd"].data_ptr(),
+                                hd_bytes=sl["b"]["hd_total"])
             reps = max(8 * S, min(256, (1 << 17) // c))
             for k in range(S):  # warm-up, one per stream
                 slots[k]["m"].plan(slots[k]["b"]["desc"])
@@ -521,7 +550,8 @@ def bench_pipeline(torch, nr, args):
                 if not torch.equal(slots[k]["h_hd"], slots[k]["want"]):
                     raise RuntimeError(f"pipelined decode differs from the lone decode (c={c}, S={S}, slot {k})")
             lat = [a.elapsed_time(b_) for a, b_ in ev]
-            row[f"streams_{S}"] = {"batches": reps, "info_gbps": round(info / wall / 1e9, 3),
+            row[f"streams_{S}"] = {"batches": reps, "grid_per_stream": full_grid // S,
+                                   "info_gbps": round(info / wall / 1e9, 3),
                                    "batches_per_s": round(reps / wall, 1),
                                    "latency_ms": {"p50": round(statistics.median(lat), 3),
                                                   "p99": round(pct(lat, 0.99), 3)}}

@@ -292,8 +292,9 @@ int ldpc_mixed_decode_async(ldpc_mixed* p, void* stream, const float* llr, int n
  * most llr_floats floats), the graph, and an optional device->host copy of hd into hd_host (at
  * most hd_bytes); the two batch copies are stream copies (DMA engines) so that replays on
  * different streams overlap them with each other's decodes.  Host buffers must be pinned
- * (hipHostMalloc).  llr_floats and hd_bytes are the sizes of llr (and of app_out, which has the
- * same layout) and of hd; they are required with a host buffer and optional (0) without one.
+ * (hipHostMalloc).  llr_floats and hd_bytes are the sizes of llr in elements of the input type
+ * (ldpc_mixed_set_input; app_out holds as many floats) and of hd in bytes; they are required with
+ * a host buffer and optional (0) without one.
  * Each replay decodes the plan current at that time, into the same buffers, and cuts the copies
  * to the LLR floats and hard-decision bytes that plan addresses; it returns LDPC_EINVAL, and
  * enqueues nothing, when that plan addresses more than the sizes given here or holds learned
@@ -306,6 +307,28 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
                        float* app_out, int* iters_out, const float* llr_host, size_t llr_floats,
                        uint8_t* hd_host, size_t hd_bytes);
 int ldpc_mixed_replay(ldpc_mixed* p, void* stream);
+
+/* Element type of the channel LLRs of later ldpc_mixed_decode_async / ldpc_mixed_capture calls on p
+ * (a capture keeps the type current at capture time).  The descriptors' llr_offset and the
+ * capture's llr_floats then count elements of that type; app_out stays float at the same offsets.
+ * The decode runs in single precision on the values converted on load:
+ *   LDPC_LLR_F32  float (the default)
+ *   LDPC_LLR_I8   signed char q, LLR = q * scale (quantize(D, x, 8) values with scale = D,
+ *                 src/host_helpers.cpp:46-58); scale must be positive and finite
+ *   LDPC_LLR_F16  IEEE binary16 bit patterns (the reference's reduce_precision<half>,
+ *                 src/setup_exp.cu:33-38)
+ * The narrower types cut the host-to-device bytes of a batch by 4x / 2x.  No reference
+ * counterpart for mixed batches. */
+#define LDPC_LLR_F32 0
+#define LDPC_LLR_I8 1
+#define LDPC_LLR_F16 2
+int ldpc_mixed_set_input(ldpc_mixed* p, int format, float scale);
+
+/* Resident workgroups of the persistent grid of later launches and captures of p (0 = every
+ * workgroup the device holds at once, the default; larger values are cut to it).  Objects that
+ * decode concurrently on different streams can each take their share of the device instead of
+ * contending with full-device grids. */
+int ldpc_mixed_set_grid(ldpc_mixed* p, int workgroups);
 
 /* Learned weights of one (kind, Z) for LDPC_FLAG_LEARNED decodes: host [n_iter][nnz_full][2]
  * (rows beyond a codeword's parity-row count are unused).  Plan again after setting them. */

@@ -23,6 +23,7 @@ EARLY_EXIT = 1
 FALLBACK_PLAIN = 2
 LEARNED = 4
 NOALLOC_SCRATCH_BYTES = 32 << 20  # LDPC_NOALLOC_SCRATCH_BYTES of ldpc_api.h
+LLR_F32, LLR_I8, LLR_F16 = 0, 1, 2  # LDPC_LLR_* element types of mixed-batch LLRs
 CRC16, CRC24A, CRC24B = 0, 1, 2
 
 if not os.path.exists(LIB_PATH):
@@ -110,6 +111,8 @@ _sig("ldpc_mixed_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c
 _sig("ldpc_mixed_capture", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_uint, _P, _P, _P, _P, ctypes.c_size_t,
      _P, ctypes.c_size_t)
 _sig("ldpc_mixed_replay", ctypes.c_int, _P, _P)
+_sig("ldpc_mixed_set_input", ctypes.c_int, _P, ctypes.c_int, ctypes.c_float)
+_sig("ldpc_mixed_set_grid", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_mixed_set_weights", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P)
 _sig("ldpc_mixed_set_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _sig("ldpc_mixed_get_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _c_int_p)
@@ -373,6 +376,15 @@ class Mixed:
 
     def replay(self, stream):
         _check(lib.ldpc_mixed_replay(self._h, stream), "ldpc_mixed_replay")
+
+    def set_input(self, fmt, scale=1.0):
+        """Element type of the LLR buffers of later decodes/captures: LLR_F32, LLR_I8 (q * scale)
+        or LLR_F16 (binary16 bit patterns)."""
+        _check(lib.ldpc_mixed_set_input(self._h, fmt, scale), "ldpc_mixed_set_input")
+
+    def set_grid(self, workgroups):
+        """Persistent-grid workgroups of later launches/captures (0: the whole device)."""
+        _check(lib.ldpc_mixed_set_grid(self._h, workgroups), "ldpc_mixed_set_grid")
 
     def set_weights(self, kind, Z, weights):
         import numpy as np

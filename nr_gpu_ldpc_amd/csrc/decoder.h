@@ -25,6 +25,9 @@ struct alignas(16) EdgeParam {
     uint32_t reserved;
 };
 
+// Element types of mixed-batch channel LLRs (LDPC_LLR_* of the C ABI).
+constexpr int kLlrF32 = 0, kLlrI8 = 1, kLlrF16 = 2;
+
 struct DecodeArgs {
     const float* llr;      // [n_cw][n_vn]
     uint8_t* hd;           // [n_cw][hd_bytes]
@@ -46,6 +49,8 @@ struct DecodeArgs {
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
     const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
+    int llr_fmt;               // mixed batches: element type of llr (kLlrF32, kLlrI8, kLlrF16)
+    float llr_scale;           // kLlrI8: LLR = q * llr_scale
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -83,6 +88,8 @@ struct MixedArgs {
     int n_iter, early_exit;
     int fallback;           // learned tiles: re-decode syndrome failures with plain min-sum
     int slot_off;           // LDS byte offset of the work-queue broadcast word
+    int llr_fmt;            // element type of llr (kLlrF32, kLlrI8, kLlrF16)
+    float llr_scale;        // kLlrI8: LLR = q * llr_scale
 };
 
 int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);

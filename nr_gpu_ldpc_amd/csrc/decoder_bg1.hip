@@ -18,13 +18,6 @@ int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds,
     return LDPC_OK;
 }
 
-template <int KIND, int ZC>
-int launch_decode_char(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_decode<KIND, ZC, false, false, false, true>), dim3(blocks), dim3(threads), lds, s, a);
-    NRLDPC_HIP(hipGetLastError());
-    return LDPC_OK;
-}
-
 template <int KIND>
 int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_decode<KIND, 0, true, true>), dim3(blocks), dim3(threads), lds, s, a);
@@ -41,7 +34,5 @@ template int launch_decode_train<kBG1>(const DecodeArgs&, int, int, size_t, hipS
 
 template int launch_decode_half<kBG1, 0>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode_half<kBG1, 384>(const DecodeArgs&, int, int, size_t, hipStream_t);
-template int launch_decode_char<kBG1, 0>(const DecodeArgs&, int, int, size_t, hipStream_t);
-template int launch_decode_char<kBG1, 384>(const DecodeArgs&, int, int, size_t, hipStream_t);
 
 }  // namespace nrldpc

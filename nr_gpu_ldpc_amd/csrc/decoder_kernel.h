@@ -190,15 +190,10 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // syndrome after the last iteration; `emit_only_ok` then writes outputs only for codewords
 // whose hard decision passes it.  Returns, per lane, whether its codeword converged (early
 // exit) or passed the final check.
-// CHAR: the reference's CHAR_PRC schedule (its default build) -- signed-char channel LLRs
-// (a.llr_i8), R and dt truncated to signed char with saturation (the oracle's rule for the
-// conversions C++ leaves undefined), the APP between iterations clamped to [-128, 127], and a
-// saturated APP (127 or -128) entering the check node without R_old; all values are integers,
-// so every single-precision sum is exact.
 // HALF: the reference's HALF_PRC schedule -- binary16 channel LLRs (a.llr_half), and R, dt and
 // the APP between iterations rounded to binary16 where the reference stores them, all
 // arithmetic in single precision.
-template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false, bool CHAR = false>
+template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false>
 __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                                              const MixedCw* __restrict__ mcw, const bool sel = true,
                                              const bool final_check = false,
@@ -207,7 +202,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && TRAIN), "mixed tiles use the plain/learned forward path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
-    static_assert(!(CHAR && (LEARNED || TRAIN || MIXED || HALF)), "the signed-char schedule is plain min-sum");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -271,11 +265,17 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     // ---- load channel LLRs
     const float* __restrict__ L = MIXED ? a.llr + mcw[cwc].llr_off : a.llr + (size_t)cwc * a.n_vn;
     const uint16_t* __restrict__ Lh = HALF ? a.llr_half + (size_t)cwc * a.n_vn : nullptr;
-    const int8_t* __restrict__ Lc = CHAR ? a.llr_i8 + (size_t)cwc * a.n_vn : nullptr;
+    // MIXED: the batch's LLR elements may be int8 (q * llr_scale) or binary16, converted to
+    // single precision on load (element offsets are the descriptors' llr_off either way)
+    const int8_t* __restrict__ Mq = MIXED ? reinterpret_cast<const int8_t*>(a.llr) + mcw[cwc].llr_off : nullptr;
+    const uint16_t* __restrict__ Mh = MIXED ? reinterpret_cast<const uint16_t*>(a.llr) + mcw[cwc].llr_off : nullptr;
     auto llr_at = [&](int n) -> float {
         if constexpr (HALF) return half_bits_to_float(Lh[n]);
-        else if constexpr (CHAR) return (float)Lc[n];
-        else return L[n];
+        else if constexpr (MIXED) {
+            if (a.llr_fmt == kLlrI8) return (float)Mq[n] * a.llr_scale;
+            if (a.llr_fmt == kLlrF16) return half_bits_to_float(Mh[n]);
+            return L[n];
+        } else return L[n];
     };
     if (!idle) {
 #pragma unroll 2
@@ -283,8 +283,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             const float v = llr_at(j * Z + r) + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
             lds_st(rRd + j * 2 * Z4, v);
             lds_st(rRd + j * 2 * Z4 + Z4, v);
-            if constexpr (CHAR) lds_stu(rAcc + j * Z4, (uint32_t)(int)v);  // integer accumulator
-            else lds_st(rAcc + j * Z4, v);
+            lds_st(rAcc + j * Z4, v);
         }
     }
     if (!idle) static_for<NND, S::COLS>([&](auto JC) {
@@ -328,9 +327,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             }
 #pragma unroll 1
             for (int j = 0; j < NND; j++) {
-                float x;
-                if constexpr (CHAR) x = (float)(int)lds_ldu(eb + 2u * NND * Z4 + 4u * er + j * Z4);
-                else x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
                 lds_stb(eb + j * Z + er, x > 0.f ? 0 : 1);
                 if (ap_out) ap_out[j * Z + er] = x;
             }
@@ -371,33 +368,11 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         __syncthreads();
     };
 
-    // CHAR schedule with compiled-in offsets (ZC > 0, single-code launches): the read-copy
-    // values of row i+1 are loaded while row i adds into the accumulator, so that their LDS
-    // latency overlaps that work instead of stalling the start of row i+1 (the read copy is not
-    // written during an iteration).  Measured at BG1 Z=384: 5.15 -> 4.90 ms per 4096
-    // codewords.  The barrier-ordered schedules gain at most 1 % from it (FP32 plain, binary16)
-    // or lose 1 % (learned), and it pushes their kernels into register spills, so they keep
-    // the in-row reads.
-    constexpr bool PF = CHAR && ZC > 0 && !MIXED;
-    float xp[S::T.max_deg > 0 ? S::T.max_deg : 1];
-    auto prefetch = [&](auto IC) {
-        constexpr int i = IC;
-        if constexpr (PF && i < ROWS) {
-            constexpr int E0 = S::T.row_start[i];
-            static_for<0, S::deg(i)>([&](auto KC) {
-                constexpr int e = E0 + KC;
-                constexpr int j = S::T.ecol[e];
-                if constexpr (j < NND) xp[KC] = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
-            });
-        }
-    };
-
     int t = 0;
     for (; t < a.n_iter; ++t) {
         // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
         // out of the iteration loop into long-lived VGPRs.
         asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW), "+v"(rRdHi));
-        if (!idle) prefetch(std::integral_constant<int, 0>{});
         CW2* w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
         CW2* w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
         static_for<0, ROWS>([&](auto IC) {
@@ -432,7 +407,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     constexpr int e = E0 + k;
                     constexpr int j = S::T.ecol[e];
                     float x;
-                    if constexpr (j < NND) x = PF ? xp[k] : read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
                     else x = app1[S::T.d1_slot[j]];
                     const float mo = (idxo == (uint32_t)k) ? a2 : a1;
                     float rv;
@@ -442,12 +417,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                         rv = as_f(as_u(mag) ^ ((wo << (32 - WO - D + k)) & 0x80000000u));
                     } else {
                         rv = as_f((as_u(mo) & 0x7fffffffu) | ((wo << (32 - WO - D + k)) & 0x80000000u));
-                        // R_old as stored: the truncated magnitude saturated to a signed char
-                        if constexpr (CHAR) rv = __builtin_amdgcn_fmed3f(rv, -128.0f, 127.0f);
                     }
-                    float qv = x - rv;
-                    // a saturated APP (127 or -128) enters as it is
-                    if constexpr (CHAR) qv = fabsf(x + 0.5f) >= 127.5f ? x : qv;
+                    const float qv = x - rv;
                     q[k] = qv;
                     ro[k] = rv;
                     const float aq = fabsf(qv);
@@ -475,7 +446,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 }
                 float n1, n2;
                 if constexpr (LEARNED) { n1 = m1; n2 = m2; }
-                else if constexpr (CHAR) { n1 = truncf(0.75f * m1); n2 = truncf(0.75f * m2); }
                 else { n1 = 0.75f * m1; n2 = 0.75f * m2; }
                 const float n1s = as_f(as_u(n1) ^ sbit), n2s = as_f(as_u(n2) ^ sbit);
                 const bool row_on = !MIXED || i < m_lane;  // MIXED: row kept by this codeword
@@ -494,11 +464,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     } else {
                         ms = eq ? n2s : n1s;
                     }
-                    float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
-                    if constexpr (CHAR) rn = __builtin_amdgcn_fmed3f(rn, -128.0f, 127.0f);
+                    const float rn = as_f(as_u(ms) ^ (as_u(q[k]) & 0x80000000u));
                     if constexpr (MIXED) dtv[k] = row_on ? rn - ro[k] : 0.0f;
                     else if constexpr (HALF) dtv[k] = round_half(rn - ro[k]);
-                    else if constexpr (CHAR) dtv[k] = __builtin_amdgcn_fmed3f(rn - ro[k], -128.0f, 127.0f);
                     else dtv[k] = rn - ro[k];
                 });
                 if constexpr (HALF) {  // R is stored in binary16; dt used the unrounded R_new
@@ -514,27 +482,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 else if constexpr (WO == 0) W[WW] = (W[WW] & 0xFFFF0000u) | fresh_w;
                 else W[WW] = (W[WW] & 0x0000FFFFu) | (fresh_w << 16);
             }  // !idle
-            // CHAR: every value is an integer, so the variable-node sums are exact in any order and
-            // need no ordering barriers: each edge adds its dt to an integer accumulator atomically.
-            if constexpr (S::T.epoch_start[i] != 0 && i != 0 && !CHAR) __syncthreads();
+            if constexpr (S::T.epoch_start[i] != 0 && i != 0) __syncthreads();
             if (idle) return;
-            prefetch(std::integral_constant<int, i + 1>{});
-            if constexpr (CHAR) {
-                static_for<0, D>([&](auto KC) {
-                    constexpr int k = KC;
-                    constexpr int e = E0 + k;
-                    constexpr int j = S::T.ecol[e];
-                    if constexpr (j < NND) {
-                        const uint32_t addr = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
-                        __hip_atomic_fetch_add((lds_u32*)(uintptr_t)addr, (uint32_t)(int)dtv[k], __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-                    } else {
-                        float& ap = app1[S::T.d1_slot[j]];
-                        ap = ap + dtv[k];
-                    }
-                });
-                return;
-            }
             // VN update, ascending block row per column
             // read-modify-write in chunks of 8 edges: loads first, then stores
             static_for<0, (D + 7) / 8>([&](auto CC) {
@@ -574,18 +523,11 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         if (!idle) {
 #pragma unroll 2
             for (int j = 0; j < NND; j++) {
-                float v;
-                if constexpr (CHAR) v = (float)(int)lds_ldu(rAcc + j * Z4);
-                else v = lds_ld(rAcc + j * Z4);
+                float v = lds_ld(rAcc + j * Z4);
                 if constexpr (HALF) {  // the APP between iterations is stored in binary16
                     if (!last) {
                         v = round_half(v);
                         lds_st(rAcc + j * Z4, v);
-                    }
-                } else if constexpr (CHAR) {  // ... or as a signed char
-                    if (!last) {
-                        v = __builtin_amdgcn_fmed3f(v, -128.0f, 127.0f);
-                        lds_stu(rAcc + j * Z4, (uint32_t)(int)v);
                     }
                 }
                 lds_st(rRd + j * 2 * Z4, v);
@@ -595,11 +537,6 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 if (!last) {
 #pragma unroll
                     for (int n = 0; n < ND1; n++) app1[n] = round_half(app1[n]);
-                }
-            } else if constexpr (CHAR) {
-                if (!last) {
-#pragma unroll
-                    for (int n = 0; n < ND1; n++) app1[n] = __builtin_amdgcn_fmed3f(app1[n], -128.0f, 127.0f);
                 }
             }
         }
@@ -697,9 +634,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     return emitted || final_ok;
 }
 
-template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false, bool CHAR = false>
+template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
-    decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF, CHAR>(a, blockIdx.x, nullptr);
+    decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF>(a, blockIdx.x, nullptr);
 }
 
 }  // namespace nrldpc

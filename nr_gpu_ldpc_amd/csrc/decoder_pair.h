@@ -63,6 +63,11 @@ __device__ __forceinline__ u32 U(s2 v) { return __builtin_bit_cast(u32, v); }
 __device__ __forceinline__ s2 smin(s2 a, s2 b) { return __builtin_elementwise_min(a, b); }
 __device__ __forceinline__ s2 smax(s2 a, s2 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ s2 sp(short v) { return (s2){v, v}; }
+typedef unsigned short u2 __attribute__((ext_vector_type(2)));
+// a - b per half, saturated at 0 (unsigned): one v_pk_sub_u16 with clamp
+__device__ __forceinline__ s2 usub_sat(s2 a, s2 b) {
+    return __builtin_bit_cast(s2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, a), __builtin_bit_cast(u2, b)));
+}
 
 extern __shared__ __attribute__((aligned(16))) unsigned char g_psmem[];
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
@@ -237,9 +242,10 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
             asm volatile("" : "+s"(mr));
             if (i >= mr) return;
             asm volatile("" : "+s"(et));
-            // rows wider than KEEP edges recompute |Q| in pass 2 instead of holding it (registers)
+            // rows of at most 12 edges hold |Q| and R_old from pass 1 to pass 2; wider ones (the
+            // register peak) recompute them there
             constexpr bool KEEP_A = D <= 12;
-            s2 q[D], aq[KEEP_A ? D : 1];
+            s2 q[D], aq[KEEP_A ? D : 1], rok[KEEP_A ? D : 1];
             s2 rod1 = sp(0);  // R_old of the row's degree-1 edge, if any
             s2 m1 = sp(1000), m2 = sp(1000);
             u32 par = 0u;
@@ -251,6 +257,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                 s2 qv;
                 if constexpr (j < NND) {
                     const s2 ro = r_get<PR::T.idx[e]>(RW);
+                    if constexpr (KEEP_A) rok[k] = ro;
                     const u32 off = off2<KIND, ZC, e, j>(et);
                     qv = q_of(ld_at(rRd, rRdHi, off), ro);
                 } else {
@@ -271,7 +278,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
             const s2 n1 = smin((m1 * sp(3)) >> 2, sp(128));
             const s2 n2 = smin((m2 * sp(3)) >> 2, sp(128));
             const s2 n1s = (n1 ^ pm) - pm, n2s = (n2 ^ pm) - pm;
-            const s2 d12 = n1s - n2s;
+            const s2 d21 = n2s - n1s;
+            const s2 m1p = m1 + sp(1);
             // pass 2, stage by stage over chunks of CH edges (dependent packed instructions issued
             // back to back cost a wait state each): R, dt, the VN update; R stored back two edges
             // per register
@@ -281,17 +289,15 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                 constexpr int K0 = CC * CH;
                 constexpr int CN = (D - K0) < CH ? (D - K0) : CH;
                 s2 v[CN], sq[CN];
-                sfor<0, CN>([&](auto KC) {  // |Q| - m1
-                    constexpr int k = K0 + KC;
-                    if constexpr (KEEP_A) v[KC] = aq[k] - m1;
-                    else v[KC] = smax(q[k], sp(0) - q[k]) - m1;
-                });
-                sfor<0, CN>([&](auto KC) {  // 0 on the edge(s) with |Q| == m1, else 1
-                    v[KC] = smin(v[KC], sp(1));
+                if constexpr (!KEEP_A)
+                    sfor<0, CN>([&](auto KC) { v[KC] = smax(q[K0 + KC], sp(0) - q[K0 + KC]); });
+                sfor<0, CN>([&](auto KC) {  // (m1 + 1) - |Q| saturated at 0: 1 on the edge(s) with |Q| == m1
+                    if constexpr (KEEP_A) v[KC] = usub_sat(m1p, aq[K0 + KC]);
+                    else v[KC] = usub_sat(m1p, v[KC]);
                     asm("" : "+v"(v[KC]));
                     sq[KC] = q[K0 + KC] >> 15;
                 });
-                sfor<0, CN>([&](auto KC) { v[KC] = v[KC] * d12 + n2s; });  // +-scaled magnitude
+                sfor<0, CN>([&](auto KC) { v[KC] = v[KC] * d21 + n1s; });  // +-scaled magnitude
                 sfor<0, CN>([&](auto KC) { v[KC] = v[KC] ^ sq[KC]; });
                 sfor<0, CN>([&](auto KC) { v[KC] = v[KC] - sq[KC]; });    // own sign
                 sfor<0, CN>([&](auto KC) { v[KC] = smin(v[KC], sp(127)); });  // R_new
@@ -299,8 +305,9 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                 sfor<0, CN>([&](auto KC) {
                     constexpr int e = E0 + K0 + KC;
                     constexpr int j = Sx::T.ecol[e];
-                    if constexpr (j < NND) dt[KC] = v[KC] - r_get<PR::T.idx[e]>(RW);
-                    else dt[KC] = v[KC] - rod1;
+                    if constexpr (j >= NND) dt[KC] = v[KC] - rod1;
+                    else if constexpr (KEEP_A) dt[KC] = v[KC] - rok[K0 + KC];
+                    else dt[KC] = v[KC] - r_get<PR::T.idx[e]>(RW);
                 });
                 sfor<0, CN>([&](auto KC) { dt[KC] = smax(dt[KC], sp(-128)); });
                 sfor<0, CN>([&](auto KC) {

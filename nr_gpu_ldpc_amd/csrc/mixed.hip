@@ -46,6 +46,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         const int kind = t->kind;
         DecodeArgs a{};
         a.llr = m.llr;
+        a.llr_fmt = m.llr_fmt;
+        a.llr_scale = m.llr_scale;
         a.hd = m.hd;
         a.app_out = m.app_out;
         a.iters_out = m.iters_out;

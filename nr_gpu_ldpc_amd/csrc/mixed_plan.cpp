@@ -57,6 +57,9 @@ struct ldpc_mixed {
     int device = -1;
     int max_cw = 0;
     int n_cu = 256, grid = 256, threads = kMixedThreads;
+    int grid_full = 256;               // resident workgroups of the whole chip (the default grid)
+    int llr_fmt = kLlrF32;             // element type of the llr buffers of later launches
+    float llr_scale = 1.0f;            // kLlrI8: LLR = q * llr_scale
     int lds = 0;                       // dynamic LDS bytes per workgroup
     std::map<int, MixedCode> codes;
     unsigned char* h_plan = nullptr;   // pinned staging: [header][tiles][codewords]
@@ -84,6 +87,7 @@ struct ldpc_mixed {
     size_t g_llr_floats = 0, g_hd_bytes = 0;  // buffer sizes given at capture (0: not given)
     int g_n_iter = 0;                         // iteration count and flags of the captured decode
     unsigned g_flags = 0;
+    size_t g_elem = 4;                        // bytes per LLR element of the captured decode
     long long plan_llr_extent = 0, plan_hd_extent = 0;  // floats / bytes the current plan touches
     std::mutex mu;
 };
@@ -119,6 +123,8 @@ static int mixed_get_code(ldpc_mixed* p, int kind, int Z, MixedCode** out) {
     *out = &res.first->second;
     return LDPC_OK;
 }
+
+static size_t llr_elem_bytes(int fmt) { return fmt == kLlrI8 ? 1 : fmt == kLlrF16 ? 2 : 4; }
 
 static int mixed_cpb(const MixedCode& mc) {
     return mc.cpb > 0 ? std::min(mc.cpb, mc.cpb_max) : mc.cpb_default;
@@ -291,6 +297,8 @@ static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flag
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.fallback = (learned && (flags & LDPC_FLAG_FALLBACK_PLAIN)) ? 1 : 0;
     a.slot_off = p->lds - 16;
+    a.llr_fmt = p->llr_fmt;
+    a.llr_scale = p->llr_scale;
     return LDPC_OK;
 }
 
@@ -342,7 +350,7 @@ int ldpc_mixed_create(ldpc_mixed** out, int device, int max_cw) {
     if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(LDPC_EHIP);
     const int occ = std::max(1, mixed_occupancy(false, p->threads, (size_t)p->lds));
-    p->grid = p->n_cu * occ;
+    p->grid = p->grid_full = p->n_cu * occ;
     if (hipHostMalloc(reinterpret_cast<void**>(&p->h_plan), p->plan_bytes, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&p->d_plan), p->plan_bytes) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&p->d_queue), 4 * sizeof(unsigned)) != hipSuccess) {
@@ -467,6 +475,7 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
     p->g_hd_bytes = hd_bytes;
     p->g_n_iter = n_iter;
     p->g_flags = flags;
+    p->g_elem = llr_elem_bytes(p->llr_fmt);
     NRLDPC_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
     return LDPC_OK;
 }
@@ -495,7 +504,7 @@ int ldpc_mixed_replay(ldpc_mixed* p, void* stream) {
                                                  hipMemcpyHostToDevice));
     NRLDPC_HIP(hipGraphExecMemcpyNodeSetParams1D(p->exec, p->n_body, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
                                                  plan_body_bytes(p), hipMemcpyHostToDevice));
-    if (lf) NRLDPC_HIP(hipMemcpyAsync(p->g_llr, p->g_llr_host, lf * sizeof(float), hipMemcpyHostToDevice, s));
+    if (lf) NRLDPC_HIP(hipMemcpyAsync(p->g_llr, p->g_llr_host, lf * p->g_elem, hipMemcpyHostToDevice, s));
     NRLDPC_HIP(hipGraphLaunch(p->exec, s));
     if (hb) NRLDPC_HIP(hipMemcpyAsync(p->g_hd_host, p->g_hd, hb, hipMemcpyDeviceToHost, s));
     NRLDPC_HIP(hipEventRecord(p->uploaded, s));
@@ -524,6 +533,25 @@ int ldpc_mixed_set_weights(ldpc_mixed* p, int kind, int Z, int n_iter, const flo
     NRLDPC_HIP(hipMemcpy(mc->d_weights, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     mc->weights_iters = n_iter;
     p->planned = false;  // tiles hold weight pointers: plan again
+    return LDPC_OK;
+}
+
+int ldpc_mixed_set_input(ldpc_mixed* p, int format, float scale) {
+    if (!p || (format != LDPC_LLR_F32 && format != LDPC_LLR_I8 && format != LDPC_LLR_F16) ||
+        (format == LDPC_LLR_I8 && !(scale > 0.0f && scale < 3.0e38f))) {
+        set_last_error("mixed input: unknown element type or a scale that is not a positive finite number");
+        return LDPC_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->llr_fmt = format;
+    p->llr_scale = format == LDPC_LLR_I8 ? scale : 1.0f;
+    return LDPC_OK;
+}
+
+int ldpc_mixed_set_grid(ldpc_mixed* p, int workgroups) {
+    if (!p || workgroups < 0) return LDPC_EINVAL;
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->grid = workgroups == 0 ? p->grid_full : std::min(workgroups, p->grid_full);
     return LDPC_OK;
 }
 

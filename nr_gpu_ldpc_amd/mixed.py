@@ -122,3 +122,10 @@ def info_errors(torch, batch, hd, bits):
     err = (got != bits[llr_off[cw] + k].to(torch.int32)).to(torch.int64)
     per_cw = torch.zeros(n, dtype=torch.int64, device=dev).index_add_(0, cw, err)
     return int(per_cw.sum()), int((per_cw > 0).sum()), per_cw
+
+
+def quantize(x, D=0.5, B=8):
+    """The reference's quantize(D, x, B) (src/host_helpers.cpp:46-58) on the host: floor(x / D)
+    clamped to [-2^(B-1), 2^(B-1) - 1], as int8 (the LDPC_LLR_I8 element type with scale D)."""
+    q = np.floor(np.asarray(x, np.float64) / D)
+    return np.clip(q, -(1 << (B - 1)), (1 << (B - 1)) - 1).astype(np.int8)

@@ -227,3 +227,70 @@ def test_mixed_learned_matches_single_code(nr, torch_gpu, flags):
     got = _mixed(nr, torch, b, llr, T, f, m)
     want = _learned_reference(nr, torch, b, llr, T, f, weights)
     _compare(b, got, want)
+
+
+@pytest.mark.parametrize("fmt", ["i8", "f16"])
+def test_mixed_narrow_inputs_equal_float_decode_of_converted_llrs(nr, oracle, torch_gpu, fmt):
+    """int8 (quantize(0.5, x, 8), LLR = 0.5 q) and binary16 channel LLRs, converted on load: the
+    decode equals the float decode of the converted values bit for bit (direct launch and a
+    replayed graph whose upload moves 1 or 2 bytes per LLR), and sampled codewords equal the
+    oracle."""
+    torch = torch_gpu
+    print(f"case mixed input {fmt}", flush=True)
+    b = mx.draw_batch(200, seed=31, sizes=[13, 52, 208, 384], ebn0=(0.5, 3.0))
+    llr, _ = mx.make_channel(torch, b, seed=32)
+    x = llr.cpu().numpy()
+    if fmt == "i8":
+        narrow = mx.quantize(x)
+        conv = narrow.astype(np.float32) * np.float32(0.5)
+        ntype, code, scale = torch.int8, nr.LLR_I8, 0.5
+    else:
+        narrow = x.astype(np.float16)
+        conv = narrow.astype(np.float32)
+        ntype, code, scale = torch.float16, nr.LLR_F16, 1.0
+    want = _mixed(nr, torch, b, torch.from_numpy(conv).cuda(), 12, nr.EARLY_EXIT)
+    m = nr.Mixed(len(b["kind"]))
+    m.set_input(code, scale)
+    d_narrow = torch.from_numpy(narrow.view(np.int8) if fmt == "i8" else narrow.view(np.int16)).cuda()
+    got = _mixed(nr, torch, b, d_narrow, 12, nr.EARLY_EXIT, m)
+    assert all(np.array_equal(g.view(np.uint8), w.view(np.uint8)) for g, w in zip(got, want))
+    # replayed graph with the narrow host buffer
+    h_in = torch.from_numpy(d_narrow.cpu().numpy()).pin_memory()
+    h_hd = torch.zeros(b["hd_total"], dtype=torch.uint8).pin_memory()
+    d_in = torch.zeros_like(d_narrow)
+    d_hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
+    s = torch.cuda.Stream()
+    m.plan(b["desc"])
+    m.capture(s.cuda_stream, d_in.data_ptr(), 12, d_hd.data_ptr(), 0, 0, flags=nr.EARLY_EXIT,
+              llr_host_ptr=h_in.data_ptr(), llr_floats=b["llr_floats"], hd_host_ptr=h_hd.data_ptr(),
+              hd_bytes=b["hd_total"])
+    m.replay(s.cuda_stream)
+    s.synchronize()
+    assert np.array_equal(h_hd.numpy(), want[0][:b["hd_total"]])
+    # the oracle on the converted values, a few codewords of each (kind, Z)
+    d = b["desc"]
+    for i in range(0, len(b["kind"]), 17):
+        a0, nv = int(d["llr_offset"][i]), int(b["n_vn"][i])
+        h0, nb = int(d["hd_offset"][i]), int(b["hd_bytes"][i])
+        oc = oracle.code(int(b["kind"][i]), int(b["Z"][i]), int(b["m_rows"][i]))
+        ohd, oapp, oit = oracle.decode(oc, conv[a0:a0 + nv][None, :], 12, early_exit=True)
+        assert np.array_equal(got[0][h0:h0 + nb], np.packbits(ohd[0], bitorder="little")), i
+        assert np.array_equal(got[1][a0:a0 + nv].view(np.uint32), oapp[0].view(np.uint32)), i
+        assert got[2][i] == oit[0], i
+
+
+def test_mixed_reduced_grid_decodes_the_same(nr, torch_gpu):
+    """A persistent grid of a quarter (and of one workgroup) of the device decodes the same."""
+    torch = torch_gpu
+    print("case mixed reduced grid", flush=True)
+    b = mx.draw_batch(300, seed=41, sizes=[16, 104, 208, 384], ebn0=(0.5, 3.0))
+    llr, _ = mx.make_channel(torch, b, seed=42)
+    want = _mixed(nr, torch, b, llr, 10, nr.EARLY_EXIT)
+    full = nr.Mixed(300).plan_info()["grid"]
+    for g in (full // 4, 1):
+        m = nr.Mixed(300)
+        m.set_grid(g)
+        m.plan(b["desc"])
+        assert m.plan_info()["grid"] == g
+        got = _mixed(nr, torch, b, llr, 10, nr.EARLY_EXIT, m)
+        assert all(np.array_equal(x.view(np.uint8), y.view(np.uint8)) for x, y in zip(got, want)), g
