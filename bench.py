@@ -420,7 +420,8 @@ def bench_stream(torch, nr, args):
                 dl = torch.zeros(max_f, dtype=dtype, device="cuda")
                 dh = torch.zeros(max_h, dtype=torch.uint8, device="cuda")
                 m.plan(drawn[0]["desc"])
-                m.capture(pstreams[k].cuda_stream, dl.data_ptr(), T, dh.data_ptr(), 0, 0, flags=flags)
+                m.capture(pstreams[k].cuda_stream, dl.data_ptr(), T, dh.data_ptr(), 0, 0, flags=flags,
+                          llr_floats=max_f, hd_bytes=max_h)
                 slots.append((m, dl, dh))
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -491,7 +492,7 @@ def bench_pipeline(torch, nr, args):
     out = {"workload": "BG1 Z=384 / BG2 Z=208 (50/50) at rates U[1/5, 8/9], Eb/N0 1.5 dB, T=10 with "
                        "early exit; per batch: plan on the host, then one replayed HIP graph (pinned "
                        "LLR upload, plan upload, decode, hard-decision download); S streams round-robin, each "
-                       "stream's persistent grid 1/S of the resident workgroups of the device",
+                       "stream's persistent grid the whole device or (shared_grid) 1/S of its resident workgroups",
            "latency_kind": "device time per batch on its stream (start to end event), under S-way overlap"}
     for c in (16, 256, 2048):
         slots = []
@@ -519,11 +520,11 @@ def bench_pipeline(torch, nr, args):
                           "want": h_hd.clone(), "info": int(b["info_bits"].sum())})
         row = {}
         full_grid = slots[0]["m"].plan_info()["grid"]
-        for S in (1, 2, 4):
-            # each stream's persistent grid takes its share of the device
+        # every stream's persistent grid the whole device, or (shared) 1/S of it
+        for S, shared in ((1, False), (2, False), (4, False), (2, True), (4, True)):
             for k in range(S):
                 sl = slots[k]
-                sl["m"].set_grid(full_grid // S)
+                sl["m"].set_grid(full_grid // S if shared else 0)
                 sl["m"].plan(sl["b"]["desc"])
                 sl["m"].capture(streams[k].cuda_stream, sl["d_llr"].data_ptr(), T, sl["d_hd"].data_ptr(), 0, 0,
                                 flags=nr.EARLY_EXIT, llr_host_ptr=sl["h_llr"].data_ptr(),
@@ -550,7 +551,8 @@ def bench_pipeline(torch, nr, args):
                 if not torch.equal(slots[k]["h_hd"], slots[k]["want"]):
                     raise RuntimeError(f"pipelined decode differs from the lone decode (c={c}, S={S}, slot {k})")
             lat = [a.elapsed_time(b_) for a, b_ in ev]
-            row[f"streams_{S}"] = {"batches": reps, "grid_per_stream": full_grid // S,
+            row[f"streams_{S}" + ("_shared_grid" if shared else "")] = {
+                                   "batches": reps, "grid_per_stream": full_grid // S if shared else full_grid,
                                    "info_gbps": round(info / wall / 1e9, 3),
                                    "batches_per_s": round(reps / wall, 1),
                                    "latency_ms": {"p50": round(statistics.median(lat), 3),

@@ -317,8 +317,10 @@ int ldpc_mixed_replay(ldpc_mixed* p, void* stream);
  *                 src/host_helpers.cpp:46-58); scale must be positive and finite
  *   LDPC_LLR_F16  IEEE binary16 bit patterns (the reference's reduce_precision<half>,
  *                 src/setup_exp.cu:33-38)
- * The narrower types cut the host-to-device bytes of a batch by 4x / 2x.  No reference
- * counterpart for mixed batches. */
+ * The narrower types cut the host-to-device bytes of a batch by 4x / 2x.  A narrow batch is
+ * converted by one small kernel into a float buffer the object owns (grown, with an allocation,
+ * by the first launch or capture that needs more; a capture sizes it to llr_floats, which narrow
+ * inputs require) and decoded from there.  No reference counterpart for mixed batches. */
 #define LDPC_LLR_F32 0
 #define LDPC_LLR_I8 1
 #define LDPC_LLR_F16 2

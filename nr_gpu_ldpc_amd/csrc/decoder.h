@@ -49,8 +49,6 @@ struct DecodeArgs {
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
     const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
-    int llr_fmt;               // mixed batches: element type of llr (kLlrF32, kLlrI8, kLlrF16)
-    float llr_scale;           // kLlrI8: LLR = q * llr_scale
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -88,9 +86,20 @@ struct MixedArgs {
     int n_iter, early_exit;
     int fallback;           // learned tiles: re-decode syndrome failures with plain min-sum
     int slot_off;           // LDS byte offset of the work-queue broadcast word
-    int llr_fmt;            // element type of llr (kLlrF32, kLlrI8, kLlrF16)
-    float llr_scale;        // kLlrI8: LLR = q * llr_scale
 };
+
+// Narrow mixed-batch LLRs (int8 q -> q * scale, or binary16) -> float, for the element count in
+// the device word *count (the plan's LLR extent, so that one captured launch serves every plan).
+struct LlrConvertArgs {
+    const void* in;
+    float* out;
+    const long long* count;
+    int fmt;
+    float scale;
+};
+int llr_convert_launch(const LlrConvertArgs& a, hipStream_t s);
+int llr_convert_graph_node(const LlrConvertArgs& a, hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
+                           hipGraphNode_t* node);
 
 int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);
 // The same launch as a kernel node of graph g after the given dependencies.

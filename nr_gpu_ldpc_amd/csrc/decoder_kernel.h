@@ -265,17 +265,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     // ---- load channel LLRs
     const float* __restrict__ L = MIXED ? a.llr + mcw[cwc].llr_off : a.llr + (size_t)cwc * a.n_vn;
     const uint16_t* __restrict__ Lh = HALF ? a.llr_half + (size_t)cwc * a.n_vn : nullptr;
-    // MIXED: the batch's LLR elements may be int8 (q * llr_scale) or binary16, converted to
-    // single precision on load (element offsets are the descriptors' llr_off either way)
-    const int8_t* __restrict__ Mq = MIXED ? reinterpret_cast<const int8_t*>(a.llr) + mcw[cwc].llr_off : nullptr;
-    const uint16_t* __restrict__ Mh = MIXED ? reinterpret_cast<const uint16_t*>(a.llr) + mcw[cwc].llr_off : nullptr;
     auto llr_at = [&](int n) -> float {
         if constexpr (HALF) return half_bits_to_float(Lh[n]);
-        else if constexpr (MIXED) {
-            if (a.llr_fmt == kLlrI8) return (float)Mq[n] * a.llr_scale;
-            if (a.llr_fmt == kLlrF16) return half_bits_to_float(Mh[n]);
-            return L[n];
-        } else return L[n];
+        else return L[n];
     };
     if (!idle) {
 #pragma unroll 2

@@ -46,8 +46,6 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         const int kind = t->kind;
         DecodeArgs a{};
         a.llr = m.llr;
-        a.llr_fmt = m.llr_fmt;
-        a.llr_scale = m.llr_scale;
         a.hd = m.hd;
         a.app_out = m.app_out;
         a.iters_out = m.iters_out;
@@ -106,6 +104,39 @@ int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, si
     kp.gridDim = dim3(grid);
     kp.blockDim = dim3(threads);
     kp.sharedMemBytes = (unsigned)lds;
+    kp.kernelParams = params;
+    kp.extra = nullptr;
+    NRLDPC_HIP(hipGraphAddKernelNode(node, g, deps, n_deps, &kp));
+    return LDPC_OK;
+}
+
+__global__ void __launch_bounds__(256) k_llr_convert(LlrConvertArgs a) {
+    const long long n = *a.count;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float v;
+        if (a.fmt == kLlrI8) v = (float)static_cast<const int8_t*>(a.in)[i] * a.scale;
+        else v = half_bits_to_float(static_cast<const uint16_t*>(a.in)[i]);
+        a.out[i] = v;
+    }
+}
+
+constexpr int kConvertBlocks = 2048;
+
+int llr_convert_launch(const LlrConvertArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_llr_convert, dim3(kConvertBlocks), dim3(256), 0, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
+int llr_convert_graph_node(const LlrConvertArgs& a, hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
+                           hipGraphNode_t* node) {
+    LlrConvertArgs args = a;
+    void* params[] = {&args};
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void*>(k_llr_convert);
+    kp.gridDim = dim3(kConvertBlocks);
+    kp.blockDim = dim3(256);
+    kp.sharedMemBytes = 0;
     kp.kernelParams = params;
     kp.extra = nullptr;
     NRLDPC_HIP(hipGraphAddKernelNode(node, g, deps, n_deps, &kp));

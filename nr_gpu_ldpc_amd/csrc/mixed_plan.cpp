@@ -60,6 +60,8 @@ struct ldpc_mixed {
     int grid_full = 256;               // resident workgroups of the whole chip (the default grid)
     int llr_fmt = kLlrF32;             // element type of the llr buffers of later launches
     float llr_scale = 1.0f;            // kLlrI8: LLR = q * llr_scale
+    float* d_conv = nullptr;           // narrow inputs converted to float (grow-only)
+    size_t conv_cap = 0;               // floats d_conv holds
     int lds = 0;                       // dynamic LDS bytes per workgroup
     std::map<int, MixedCode> codes;
     unsigned char* h_plan = nullptr;   // pinned staging: [header][tiles][codewords]
@@ -246,7 +248,9 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
     }
     p->plan_llr_extent = le;
     p->plan_hd_extent = he;
+    // header: tile count (read by the decode) and LLR extent (read by the narrow-input conversion)
     *reinterpret_cast<int*>(p->h_plan) = (int)tiles.size();
+    *reinterpret_cast<long long*>(p->h_plan + 8) = le;
     p->n_tiles = (int)tiles.size();
     p->n_cw = n;
     p->plan_trained_T = trained_T;
@@ -297,8 +301,6 @@ static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flag
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.fallback = (learned && (flags & LDPC_FLAG_FALLBACK_PLAIN)) ? 1 : 0;
     a.slot_off = p->lds - 16;
-    a.llr_fmt = p->llr_fmt;
-    a.llr_scale = p->llr_scale;
     return LDPC_OK;
 }
 
@@ -306,15 +308,42 @@ static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flag
 static size_t plan_head_bytes(const ldpc_mixed* p) { return p->tiles_off + (size_t)p->n_tiles * sizeof(MixedTile); }
 static size_t plan_body_bytes(const ldpc_mixed* p) { return (size_t)std::max(p->n_cw, 1) * sizeof(MixedCw); }
 
-// Enqueue the plan upload and the kernel on stream s.
+// Float buffer for narrow inputs of at least n floats (grow-only; a launch still reading the old
+// one is waited for before it is freed).
+static int conv_reserve(ldpc_mixed* p, size_t n) {
+    if (p->conv_cap >= n) return LDPC_OK;
+    int st = wait_upload(p);
+    if (st) return st;
+    if (p->d_conv) NRLDPC_HIP(hipFree(p->d_conv));
+    p->d_conv = nullptr;
+    p->conv_cap = 0;
+    NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_conv), std::max<size_t>(n, 1) * sizeof(float)));
+    p->conv_cap = n;
+    return LDPC_OK;
+}
+
+static LlrConvertArgs convert_args(const ldpc_mixed* p, const void* narrow) {
+    LlrConvertArgs c{};
+    c.in = narrow;
+    c.out = p->d_conv;
+    c.count = reinterpret_cast<const long long*>(p->d_plan + 8);
+    c.fmt = p->llr_fmt;
+    c.scale = p->llr_scale;
+    return c;
+}
+
+// Enqueue the plan upload, the narrow-input conversion (if any) and the kernel on stream s.
 static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_iter, unsigned flags, uint8_t* hd,
                          float* app_out, int* iters_out) {
+    const bool narrow = p->llr_fmt != kLlrF32;
+    int st;
+    if (narrow && (st = conv_reserve(p, (size_t)p->plan_llr_extent))) return st;
     MixedArgs a;
-    int st = mixed_args(p, llr, n_iter, flags, hd, app_out, iters_out, &a);
-    if (st) return st;
+    if ((st = mixed_args(p, narrow ? p->d_conv : llr, n_iter, flags, hd, app_out, iters_out, &a))) return st;
     NRLDPC_HIP(hipMemcpyAsync(p->d_plan, p->h_plan, plan_head_bytes(p), hipMemcpyHostToDevice, s));
     NRLDPC_HIP(hipMemcpyAsync(p->d_plan + p->cws_off, p->h_plan + p->cws_off, plan_body_bytes(p),
                               hipMemcpyHostToDevice, s));
+    if (narrow && (st = llr_convert_launch(convert_args(p, llr), s))) return st;
     return mixed_launch((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, s);
 }
 
@@ -379,6 +408,7 @@ int ldpc_mixed_destroy(ldpc_mixed* p) {
             if (c.second.d_weights) (void)hipFree(c.second.d_weights);
         }
         if (p->d_plan) (void)hipFree(p->d_plan);
+        if (p->d_conv) (void)hipFree(p->d_conv);
         if (p->d_queue) (void)hipFree(p->d_queue);
         if (p->h_plan) (void)hipHostFree(p->h_plan);
     } else {
@@ -442,11 +472,17 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
         set_last_error("mixed capture: a host buffer needs its size");
         return LDPC_EINVAL;
     }
+    const bool narrow = p->llr_fmt != kLlrF32;
+    if (narrow && !llr_floats) {
+        set_last_error("mixed capture: narrow LLR inputs need llr_floats (the capacity of llr in elements)");
+        return LDPC_EINVAL;
+    }
     NRLDPC_HIP(hipSetDevice(p->device));
-    MixedArgs a;
-    int st = mixed_args(p, llr, n_iter, flags, hd, app_out, iters_out, &a);
-    if (st) return st;
+    int st;
     if ((st = check_plan_fits(p, llr_floats, hd_bytes))) return st;
+    if (narrow && (st = conv_reserve(p, llr_floats))) return st;
+    MixedArgs a;
+    if ((st = mixed_args(p, narrow ? p->d_conv : llr, n_iter, flags, hd, app_out, iters_out, &a))) return st;
     if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
     if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
     p->n_head = p->n_body = nullptr;
@@ -463,8 +499,14 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
                                        p->cws_off, hipMemcpyHostToDevice));
     NRLDPC_HIP(hipGraphAddMemcpyNode1D(&p->n_body, g, &p->n_head, 1, p->d_plan + p->cws_off, p->h_plan + p->cws_off,
                                        (size_t)p->max_cw * sizeof(MixedCw), hipMemcpyHostToDevice));
+    hipGraphNode_t before = p->n_body;
+    if (narrow) {  // the float copy of this replay's LLRs, as many as the current plan addresses
+        hipGraphNode_t conv = nullptr;
+        if ((st = llr_convert_graph_node(convert_args(p, llr), g, &p->n_body, 1, &conv))) return st;
+        before = conv;
+    }
     if ((st = mixed_graph_node((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, g,
-                               &p->n_body, 1, &kern)))
+                               &before, 1, &kern)))
         return st;
     (void)kern;
     p->g_llr_host = llr_host;
@@ -619,6 +661,8 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
         return LDPC_ENOMEM;
     }
     const int saved = mc->cpb;
+    const int saved_fmt = p->llr_fmt;  // the candidates are timed on float LLRs
+    p->llr_fmt = kLlrF32;
     int best = 1;
     float best_us = 3.4e38f;
     for (size_t c = 0; c < cand.size() && st == LDPC_OK; c++) {
@@ -645,6 +689,7 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
     }
     (void)hipDeviceSynchronize();
     cleanup();
+    p->llr_fmt = saved_fmt;
     p->planned = false;
     if (st) { mc->cpb = saved; return st; }
     mc->cpb = best;
