@@ -112,6 +112,16 @@ int ctx_build_device(ldpc_ctx* x) {
     if ((st = upload(x->prog.step_start, &x->d_step_start))) return st;
     if ((st = upload(x->prog.term_col, &x->d_term_col))) return st;
     if ((st = upload(x->prog.term_shift, &x->d_term_shift))) return st;
+    // degree-1 APP slots of the slim layout: one workgroup slot per CU
+    if (decoder_slim_cpb(c.kind, c.Z) > 0) {
+        int cus = 0;
+        NRLDPC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, x->device));
+        if (cus > 0) {
+            NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&x->d_d1), (size_t)cus * decoder_slim_slot_bytes(c.kind, c.Z)));
+            NRLDPC_HIP(hipEventCreateWithFlags(&x->d1_done, hipEventDisableTiming));
+            x->d1_slots = cus;
+        }
+    }
     return LDPC_OK;
 }
 
@@ -186,8 +196,12 @@ int ldpc_ctx_destroy(ldpc_ctx* x) {
         (void)hipEventSynchronize(x->scratch_done);
         (void)hipEventDestroy(x->scratch_done);
     }
+    if (x->d1_done) {
+        (void)hipEventSynchronize(x->d1_done);
+        (void)hipEventDestroy(x->d1_done);
+    }
     void* ptrs[] = {x->d_etab, x->d_row_start, x->d_edge_col, x->d_edge_vm, x->d_step_target,
-                    x->d_step_start, x->d_term_col, x->d_term_shift, x->d_weights, x->d_scratch};
+                    x->d_step_start, x->d_term_col, x->d_term_shift, x->d_weights, x->d_scratch, x->d_d1};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete x;
@@ -216,13 +230,15 @@ int ldpc_ctx_info(const ldpc_ctx* x, ldpc_code_info* info) {
 
 int ldpc_set_launch(ldpc_ctx* x, int cpb) {
     if (!x || cpb < 0) return LDPC_EINVAL;
-    if (cpb == 0) { x->cpb = x->cpb_default; return LDPC_OK; }
+    if (cpb == 0) { x->cpb = x->cpb_default; x->cpb_set = false; return LDPC_OK; }
     const size_t lds = decoder_lds_bytes_per_cw(x->code.kind, x->code.Z);
-    if (cpb * x->code.Z > kMaxBlock || cpb * lds + 16 > (size_t)max_lds_bytes()) {
+    const bool slim = x->d_d1 != nullptr && cpb == decoder_slim_cpb(x->code.kind, x->code.Z);
+    if (!slim && (cpb * x->code.Z > kMaxBlock || cpb * lds + 16 > (size_t)max_lds_bytes())) {
         set_last_error("cw_per_block exceeds the workgroup size or LDS limit");
         return LDPC_EINVAL;
     }
     x->cpb = cpb;
+    x->cpb_set = true;
     return LDPC_OK;
 }
 

@@ -16,6 +16,13 @@ struct ldpc_ctx {
     nrldpc::CodeDesc code;
     nrldpc::EncodeProgram prog;
     int cpb = 1, cpb_default = 1;  // codewords per workgroup
+    bool cpb_set = false;          // cpb chosen by ldpc_set_launch (else chosen per batch)
+    // Slim decoder layout (decoder_slim_cpb > 0): the degree-1 APP slots of its persistent grid,
+    // one per CU, and the event that orders the launches sharing them.
+    float* d_d1 = nullptr;
+    int d1_slots = 0;
+    hipEvent_t d1_done = nullptr;
+    std::mutex d1_mu;
     // device tables
     nrldpc::EdgeParam* d_etab = nullptr;
     int* d_row_start = nullptr;

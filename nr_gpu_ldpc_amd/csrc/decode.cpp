@@ -1,6 +1,8 @@
 // decode.cpp -- host side of the decode entry points.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "context.h"
 #include "decoder.h"
 #include "status.h"
@@ -33,7 +35,28 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     a.list = list;
     a.list_count = list_count;
     a.fail_out = fail_out;
-    return decoder_launch(c.kind, weights != nullptr, a, static_cast<hipStream_t>(stream));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    // The slim layout: explicitly chosen (ldpc_set_launch with its codewords per workgroup), or
+    // by batch size when the launch shape is left to the context.
+    const int slim_cpb = x->d_d1 ? decoder_slim_cpb(c.kind, c.Z) : 0;
+    const bool slim = slim_cpb > 0 && (x->cpb_set ? x->cpb == slim_cpb
+                                                  : decoder_slim_pays(n_cw, x->d1_slots, slim_cpb, x->cpb));
+    if (!slim) return decoder_launch(c.kind, weights != nullptr, a, s);
+    a.cpb = slim_cpb;
+    a.lds_cw_bytes = (int)decoder_slim_lds_bytes_per_cw(c.kind, c.Z);
+    a.d1_scratch = x->d_d1;
+    const int grid = std::min((n_cw + slim_cpb - 1) / slim_cpb, x->d1_slots);
+    // Launches on one context share its slots: each waits for the previous one (on any stream).
+    // Under stream capture the wait is not recorded: the captured graph orders its own nodes,
+    // and graphs that share one context must not run concurrently.
+    std::lock_guard<std::mutex> lk(x->d1_mu);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    NRLDPC_HIP(hipStreamIsCapturing(s, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (!capturing) NRLDPC_HIP(hipStreamWaitEvent(s, x->d1_done, 0));
+    const int st = decoder_launch_slim(c.kind, weights != nullptr, a, grid, s);
+    if (!capturing) NRLDPC_HIP(hipEventRecord(x->d1_done, s));
+    return st;
 }
 
 int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,

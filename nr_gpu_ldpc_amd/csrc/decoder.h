@@ -9,6 +9,7 @@
 namespace nrldpc {
 
 constexpr int kMaxBlock = 512;          // threads per workgroup (2 waves per SIMD)
+constexpr int kSlimMaxBlock = 768;      // slim-layout kernel: up to 12 wavefronts (3 per SIMD)
 constexpr int kMaxLdsBytes = 160 * 1024; // gfx950 LDS per CU (upper bound; see max_lds_bytes)
 
 // Per base-graph edge, in row-major edge order (four dwords, read through the scalar cache).
@@ -49,6 +50,8 @@ struct DecodeArgs {
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
     const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
+    float* d1_scratch;         // two-workgroups-per-CU kernel: APP of the degree-1 columns,
+                               // [gridDim.x][cpb][ND1][Z] floats (one slot per resident workgroup)
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -109,6 +112,18 @@ int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, si
 int mixed_occupancy(bool learned, int threads, size_t lds);
 
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
+// The slim layout (k_decode_slim): a single LDS read copy and the degree-1 APPs in a device
+// slot per resident workgroup, so that a workgroup of decoder_slim_cpb codewords (12 wavefronts
+// at BG1 Z=384) fits one CU at three wavefronts per SIMD.  Compiled for BG1 Z=384 only
+// (decoder_slim_cpb returns 0 elsewhere).
+int decoder_slim_cpb(int kind, int Z);
+size_t decoder_slim_lds_bytes_per_cw(int kind, int Z);
+size_t decoder_slim_slot_bytes(int kind, int Z);  // a.d1_scratch bytes per workgroup slot
+// Persistent launch of `grid` workgroups (a.d1_scratch holds grid slots).
+int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s);
+// Whether the slim launch is expected to finish n_cw codewords sooner than the default layout on
+// a device with `cus` compute units (per-round costs measured on MI355X).
+bool decoder_slim_pays(int n_cw, int cus, int slim_cpb, int default_cpb);
 int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
 // The reference's HALF_PRC schedule (binary16 LLRs in a.llr_half).

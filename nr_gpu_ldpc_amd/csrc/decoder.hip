@@ -16,6 +16,8 @@ int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds
 
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
+template <int KIND, int ZC, bool LEARNED>
+int launch_decode_slim(const DecodeArgs& a, int grid, int threads, size_t lds, hipStream_t s);
 template <int KIND, int ZC>
 int launch_decode_char_pair(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 
@@ -41,6 +43,48 @@ size_t decoder_lds_bytes_per_cw(int kind, int Z) {
     else if (kind == kBG2) nnd = Structure<kBG2>::NND;
     else nnd = Structure<kWifi>::NND;
     return ((size_t)3 * nnd * 4 * Z + 16 + 15) & ~(size_t)15;
+}
+
+int decoder_slim_cpb(int kind, int Z) { return kind == kBG1 && Z == kBG1Special ? 2 : 0; }
+
+size_t decoder_slim_lds_bytes_per_cw(int kind, int Z) {
+    int nnd = 0;
+    if (kind == kBG1) nnd = Structure<kBG1>::NND;
+    else if (kind == kBG2) nnd = Structure<kBG2>::NND;
+    else nnd = Structure<kWifi>::NND;
+    return ((size_t)2 * nnd * 4 * Z + 16 + 15) & ~(size_t)15;
+}
+
+size_t decoder_slim_slot_bytes(int kind, int Z) {
+    int nd1 = 0;
+    if (kind == kBG1) nd1 = Structure<kBG1>::ND1;
+    else if (kind == kBG2) nd1 = Structure<kBG2>::ND1;
+    return (size_t)decoder_slim_cpb(kind, Z) * nd1 * Z * sizeof(float);
+}
+
+// One round of the slim kernel (every CU decoding slim_cpb codewords) takes this many times as
+// long as one round of the default layout (every CU decoding default_cpb codewords): BG1 Z=384
+// T=10, 4096 codewords, 3.72 ms in 8 rounds against 4.29 ms in 16 on MI355X.
+constexpr double kSlimRoundCost = 1.73;
+
+bool decoder_slim_pays(int n_cw, int cus, int slim_cpb, int default_cpb) {
+    if (slim_cpb <= 0 || cus <= 0 || n_cw <= 0) return false;
+    const long long per_round_d = (long long)cus * default_cpb, per_round_s = (long long)cus * slim_cpb;
+    const long long rounds_d = (n_cw + per_round_d - 1) / per_round_d;
+    const long long rounds_s = (n_cw + per_round_s - 1) / per_round_s;
+    return (double)rounds_s * kSlimRoundCost < (double)rounds_d;
+}
+
+int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s) {
+    const int threads = a.cpb * a.Z;
+    const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;  // + workgroup flag slot
+    if (a.cpb != decoder_slim_cpb(kind, a.Z) || threads > kSlimMaxBlock || lds > (size_t)max_lds_bytes() ||
+        grid < 1 || !a.d1_scratch) {
+        set_last_error("slim decoder launch: unsupported shape");
+        return LDPC_EINVAL;
+    }
+    return learned ? launch_decode_slim<kBG1, kBG1Special, true>(a, grid, threads, lds, s)
+                   : launch_decode_slim<kBG1, kBG1Special, false>(a, grid, threads, lds, s);
 }
 
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s) {

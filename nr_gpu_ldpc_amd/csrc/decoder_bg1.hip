@@ -18,6 +18,13 @@ int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds,
     return LDPC_OK;
 }
 
+template <int KIND, int ZC, bool LEARNED>
+int launch_decode_slim(const DecodeArgs& a, int grid, int threads, size_t lds, hipStream_t s) {
+    hipLaunchKernelGGL((k_decode_slim<KIND, ZC, LEARNED>), dim3(grid), dim3(threads), lds, s, a);
+    NRLDPC_HIP(hipGetLastError());
+    return LDPC_OK;
+}
+
 template <int KIND>
 int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_decode<KIND, 0, true, true>), dim3(blocks), dim3(threads), lds, s, a);
@@ -29,6 +36,9 @@ template int launch_decode<kBG1, 0, false>(const DecodeArgs&, int, int, size_t, 
 template int launch_decode<kBG1, 0, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG1, 384, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG1, 384, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
+
+template int launch_decode_slim<kBG1, 384, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int launch_decode_slim<kBG1, 384, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
 
 template int launch_decode_train<kBG1>(const DecodeArgs&, int, int, size_t, hipStream_t);
 

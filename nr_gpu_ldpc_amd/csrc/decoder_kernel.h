@@ -102,6 +102,9 @@ __device__ __forceinline__ uint32_t block_any2(bool a, bool b, uint32_t slot) {
 __device__ __forceinline__ float round_half(float x) { return (float)(_Float16)x; }
 __device__ __forceinline__ float half_bits_to_float(uint16_t u) { return (float)__builtin_bit_cast(_Float16, u); }
 
+// Dword 3 of a raw buffer resource on gfx9 (no format conversion, bounds by byte count).
+constexpr int kBufferDword3 = 0x00020000;
+
 // Per-edge table entries are read through the constant address space so that the compiler
 // issues scalar (s_load) reads of these wave-uniform values.
 using CEdge = const EdgeParam __attribute__((address_space(4)));
@@ -193,7 +196,11 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
 // HALF: the reference's HALF_PRC schedule -- binary16 channel LLRs (a.llr_half), and R, dt and
 // the APP between iterations rounded to binary16 where the reference stores them, all
 // arithmetic in single precision.
-template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false>
+// SLIM: the state of one codeword cut to fit two workgroups per CU -- a single read copy
+// [NND][Z] (the circulant read selects between two bases, like the accumulator update) and the
+// APP of the degree-1 columns in the workgroup's slot of a.d1_scratch instead of VGPRs.  Same
+// arithmetic in the same order, so the results are bit-identical to the default layout.
+template <int KIND, int ZC, bool LEARNED, bool TRAIN, bool MIXED, bool HALF = false, bool SLIM = false>
 __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                                              const MixedCw* __restrict__ mcw, const bool sel = true,
                                              const bool final_check = false,
@@ -202,6 +209,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && TRAIN), "mixed tiles use the plain/learned forward path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
+    static_assert(!(SLIM && (TRAIN || MIXED || HALF)), "the slim layout serves the plain and learned decodes");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -243,12 +251,27 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     const bool idle = MIXED && (int)(__builtin_amdgcn_readfirstlane(tid) & ~63) >= a.cpb * Z;
     const uint32_t Z4 = 4u * Z;
     const uint32_t base = lds_base() + cwl * a.lds_cw_bytes;
+    constexpr uint32_t RDN = SLIM ? 1u : 2u;       // copies of each sub-column in the read copy
     uint32_t rRd = base + 4u * r;                  // read copy, block column 0
-    uint32_t rAcc = base + 2u * NND * Z4 + 4u * r; // accumulator, block column 0
+    uint32_t rAcc = base + RDN * NND * Z4 + 4u * r; // accumulator, block column 0
     uint32_t rAccW = rAcc - Z4;                    // same, for lanes whose (r+V) wraps
-    uint32_t rRdHi = rRd + 0x10000u;               // second base: immediates are 16-bit
+    uint32_t rRdHi = SLIM ? rRd - Z4 : rRd + 0x10000u;  // SLIM: read copy for wrapping lanes;
+                                                   // else a second base (16-bit immediates)
     const uint32_t rHi = ((uint32_t)r << 16) | 0xFFFFu;  // compared with EdgeParam::acc
-    const uint32_t flagAddr = base + 3u * NND * Z4;
+    const uint32_t flagAddr = base + (RDN + 1u) * NND * Z4;
+    // SLIM: this lane's degree-1 APPs, slot s at byte d1v + s * 4Z of the workgroup's scratch
+    // slot, addressed through a buffer resource (wave-uniform base, one lane offset for all
+    // slots: no per-slot 64-bit address is formed or kept)
+    const __amdgpu_buffer_rsrc_t d1rs = __builtin_amdgcn_make_buffer_rsrc(
+        SLIM ? a.d1_scratch + (size_t)blockIdx.x * a.cpb * ND1 * Z : nullptr, 0, SLIM ? a.cpb * ND1 * Z * 4 : 0,
+        kBufferDword3);
+    const uint32_t d1v = SLIM ? (uint32_t)(cwl * ND1 * Z + r) * 4u : 0u;
+    auto d1_ld = [&](int slot) -> float {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(d1rs, d1v, slot * Z * 4, 0));
+    };
+    auto d1_st = [&](int slot, float v) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), d1rs, d1v, slot * Z * 4, 0);
+    };
     const uint32_t wgFlag = lds_base() + (uint32_t)a.cpb * a.lds_cw_bytes;  // workgroup flag slot
     CEdge* et = (CEdge*)(a.etab);
     const int m_rows = a.m_rows;
@@ -256,7 +279,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     constexpr int NW = S::NW;
     float A1[ROWS], A2[ROWS];  // plain: fl(0.75 m1), fl(0.75 m2); learned: m1, m2
     uint32_t W[NW];            // per row: sign of R per edge + arg-min index (see Structure)
-    float app1[ND1];           // APP of the degree-1 columns fed by this lane's rows
+    float app1[SLIM ? 1 : ND1];  // APP of the degree-1 columns fed by this lane's rows
 #pragma unroll
     for (int i = 0; i < ROWS; i++) { A1[i] = 0.f; A2[i] = 0.f; }
 #pragma unroll
@@ -273,8 +296,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
 #pragma unroll 2
         for (int j = 0; j < NND; j++) {
             const float v = llr_at(j * Z + r) + 0.0f;  // -0.0 -> +0.0 (see the sign rule below)
-            lds_st(rRd + j * 2 * Z4, v);
-            lds_st(rRd + j * 2 * Z4 + Z4, v);
+            lds_st(rRd + j * RDN * Z4, v);
+            if constexpr (!SLIM) lds_st(rRd + j * 2 * Z4 + Z4, v);
             lds_st(rAcc + j * Z4, v);
         }
     }
@@ -287,7 +310,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             if constexpr (ZC > 0) c = r + edge_vm<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = llr_at(j * Z + c) + 0.0f;
+            if constexpr (SLIM) d1_st(S::T.d1_slot[j], llr_at(j * Z + c) + 0.0f);
+            else if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = llr_at(j * Z + c) + 0.0f;
             else app1[S::T.d1_slot[j]] = 0.0f;
         }
     });
@@ -319,7 +343,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             }
 #pragma unroll 1
             for (int j = 0; j < NND; j++) {
-                const float x = lds_ld(eb + 2u * NND * Z4 + 4u * er + j * Z4);
+                const float x = lds_ld(eb + RDN * NND * Z4 + 4u * er + j * Z4);
                 lds_stb(eb + j * Z + er, x > 0.f ? 0 : 1);
                 if (ap_out) ap_out[j * Z + er] = x;
             }
@@ -332,7 +356,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     if constexpr (ZC > 0) c = er + edge_vm<KIND, ZC>(e);
                     else c = er + (int)ee[e].off_rd;
                     if (c >= Z) c -= Z;
-                    const float x = app1[S::T.d1_slot[j]];
+                    float x;
+                    if constexpr (SLIM) x = d1_ld(S::T.d1_slot[j]);
+                    else x = app1[S::T.d1_slot[j]];
                     lds_stb(eb + j * Z + c, x > 0.f ? 0 : 1);
                     if (ap_out) ap_out[j * Z + c] = x;
                 }
@@ -384,6 +410,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             uint32_t rrHi = rHi;
             asm volatile("" : "+v"(rr), "+v"(rrHi));
             float dtv[D];
+            float xd1[D];  // SLIM: APP_old of the row's degree-1 edges
             if (!idle) {
                 float q[D], ro[D];
                 float m1 = 1000.0f, m2 = 1000.0f;
@@ -399,8 +426,15 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     constexpr int e = E0 + k;
                     constexpr int j = S::T.ecol[e];
                     float x;
-                    if constexpr (j < NND) x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
-                    else x = app1[S::T.d1_slot[j]];
+                    if constexpr (j < NND) {
+                        if constexpr (SLIM) x = lds_ld(acc_addr<ZC, KIND, e, j>(rr, rrHi, rRd, rRdHi, et));
+                        else x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    } else if constexpr (SLIM) {
+                        x = d1_ld(S::T.d1_slot[j]);
+                        xd1[k] = x;
+                    } else {
+                        x = app1[S::T.d1_slot[j]];
+                    }
                     const float mo = (idxo == (uint32_t)k) ? a2 : a1;
                     float rv;
                     if constexpr (LEARNED) {
@@ -498,6 +532,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     constexpr int j = S::T.ecol[e];
                     if constexpr (j < NND) {
                         if (!MIXED || !dead) lds_st(addr[KC], cur[KC] + dtv[k]);
+                    } else if constexpr (SLIM) {
+                        d1_st(S::T.d1_slot[j], xd1[k] + dtv[k]);
                     } else {
                         float& ap = app1[S::T.d1_slot[j]];
                         ap = ap + dtv[k];
@@ -522,8 +558,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                         lds_st(rAcc + j * Z4, v);
                     }
                 }
-                lds_st(rRd + j * 2 * Z4, v);
-                lds_st(rRd + j * 2 * Z4 + Z4, v);
+                lds_st(rRd + j * RDN * Z4, v);
+                if constexpr (!SLIM) lds_st(rRd + j * 2 * Z4 + Z4, v);
             }
             if constexpr (HALF) {
                 if (!last) {
@@ -568,8 +604,14 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 static_for<0, D>([&](auto KC) {
                     constexpr int e = E0 + KC;
                     constexpr int j = S::T.ecol[e];
-                    if constexpr (j < NND) xs[KC] = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
-                    else xs[KC] = app1[S::T.d1_slot[j]];
+                    if constexpr (j < NND) {
+                        if constexpr (SLIM) xs[KC] = lds_ld(acc_addr<ZC, KIND, e, j>(r, rHi, rRd, rRdHi, et));
+                        else xs[KC] = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                    } else if constexpr (SLIM) {
+                        xs[KC] = d1_ld(S::T.d1_slot[j]);
+                    } else {
+                        xs[KC] = app1[S::T.d1_slot[j]];
+                    }
                 });
                 // All of the row's LDS reads are issued before the first compare: under the
                 // kernel's register pressure the scheduler would otherwise wait for each read
@@ -629,6 +671,19 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
 template <int KIND, int ZC, bool LEARNED, bool TRAIN = false, bool HALF = false>
 __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
     decode_block<KIND, ZC, LEARNED, TRAIN, false, HALF>(a, blockIdx.x, nullptr);
+}
+
+// The slim layout with a persistent grid: every workgroup owns one slot of a.d1_scratch and
+// decodes blocks blockIdx.x, blockIdx.x + gridDim.x, ...  Register budget for three wavefronts
+// per SIMD (two 6-wavefront workgroups per CU, or one of 12).
+template <int KIND, int ZC, bool LEARNED>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kSlimMaxBlock), amdgpu_waves_per_eu(3)))
+k_decode_slim(DecodeArgs a) {
+    const int nblk = (a.n_cw + a.cpb - 1) / a.cpb;
+    for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        decode_block<KIND, ZC, LEARNED, false, false, false, true>(a, blk, nullptr);
+        __syncthreads();  // the next block's loads overwrite this block's LDS
+    }
 }
 
 }  // namespace nrldpc
