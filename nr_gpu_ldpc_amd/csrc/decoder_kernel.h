@@ -136,6 +136,16 @@ constexpr int edge_vm(int e) {
 }
 
 
+// The degree-1 slot that block row i feeds, or -1 (none, or more than one).
+template <int KIND>
+constexpr int row_d1_slot(int i) {
+    using S = Structure<KIND>;
+    int slot = -1, n = 0;
+    for (int e = S::T.row_start[i]; e < S::T.row_start[i + 1]; e++)
+        if (S::T.d1_slot[S::T.ecol[e]] >= 0) { slot = S::T.d1_slot[S::T.ecol[e]]; n++; }
+    return n == 1 ? slot : -1;
+}
+
 // Read copy of block column J (< NND) at sub-column (r + V) for edge E of this lane.
 template <int ZC, int KIND, int E, int J, class ET>
 __device__ __forceinline__ float read_copy(uint32_t rRd, uint32_t rRdHi, ET et) {
@@ -409,6 +419,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             int rr = r;
             uint32_t rrHi = rHi;
             asm volatile("" : "+v"(rr), "+v"(rrHi));
+            // SLIM: the row's degree-1 APP is loaded first, ahead of the LDS reads
+            float d1x = 0.0f;
+            if constexpr (SLIM && row_d1_slot<KIND>(i) >= 0) d1x = d1_ld(row_d1_slot<KIND>(i));
             float dtv[D];
             float xd1[D];  // SLIM: APP_old of the row's degree-1 edges
             if (!idle) {
@@ -430,7 +443,8 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                         if constexpr (SLIM) x = lds_ld(acc_addr<ZC, KIND, e, j>(rr, rrHi, rRd, rRdHi, et));
                         else x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
                     } else if constexpr (SLIM) {
-                        x = d1_ld(S::T.d1_slot[j]);
+                        if constexpr (row_d1_slot<KIND>(i) == S::T.d1_slot[j]) x = d1x;
+                        else x = d1_ld(S::T.d1_slot[j]);
                         xd1[k] = x;
                     } else {
                         x = app1[S::T.d1_slot[j]];
