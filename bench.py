@@ -43,8 +43,9 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 # Non-packed 32-bit VALU lane-ops/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz.
 PEAK_VALU_GOPS = 256 * 64 * 2.4
 VALU_OPS_PER_EDGE = 8  # convention: sub, min, med3, sign, select, sign-apply, dt-sub, APP-add
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_decode_bg1_z384_t10_b4096_traffic.json")
-ISSUE_FILE = os.path.join(ROOT, "profiles", "k_decode_bg1_z384_t10_b4096_sq.json")
+# counter summaries of the kernel that decodes the headline batch (the slim layout)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_decode_slim_bg1_z384_t10_b4096_traffic.json")
+ISSUE_FILE = os.path.join(ROOT, "profiles", "k_decode_slim_bg1_z384_t10_b4096_sq.json")
 # A CU issues at most one 64-lane VALU instruction per clock (4 SIMDs x 16 lanes).
 PEAK_VALU_ISSUE = 256 * 2.4e9
 TUNING_FILE = os.path.join(ROOT, "nr_gpu_ldpc_amd", "tuning", "mixed_gfx950.txt")
@@ -199,7 +200,7 @@ def bench_headline(torch, nr, args):
     char_fe = int(counts.tolist()[1]) - frame_err
     res = {
         "value": round(gbps, 4), "ms_per_step": round(ms, 4),
-        "cw_per_block": code.cw_per_block,
+        "launch_shape": dict(zip(("cw_per_block", "workgroups"), code.launch_shape(BATCH))),
         "latency_ms": {"p50": round(statistics.median(per), 4), "p99": round(pct(per, 0.99), 4),
                        "kind": "device time per batch of 4096"},
         "fer": round(frame_err / BATCH, 6), "ber": bit_err / (BATCH * code.K),
@@ -210,6 +211,7 @@ def bench_headline(torch, nr, args):
                      "note": "the reference's CHAR_PRC schedule (its default build): signed-char LLRs "
                              "(quantize(0.5, x, 8)), R, dt and inter-iteration APP"},
         "roofline": {
+            "kernel": (tsrc or {}).get("kernel"),
             "bound": "hbm",
             "achieved": round(hbm_gbs, 1),
             "peak": PEAK_HBM_GBS,
@@ -219,6 +221,7 @@ def bench_headline(torch, nr, args):
             "per_launch": {"algorithmic_bytes": alg_bytes, "bytes_per_codeword": 4 * code.n_vn + code.hd_bytes,
                            "codewords": BATCH, "edge_iterations": BATCH * edges * N_ITER},
             "traffic_source": (tsrc or {}).get("source"),
+            "traffic_note": (tsrc or {}).get("note"),
             "valu": {"note": "the binding limit: min-sum is compare/select/add work on the VALU",
                      "achieved": round(valu_gops, 1), "peak": round(PEAK_VALU_GOPS, 1), "unit": "Gop/s",
                      "frac": round(valu_gops / PEAK_VALU_GOPS, 4), "ops_per_edge": VALU_OPS_PER_EDGE,
@@ -762,7 +765,7 @@ def main():
         "config": {"workload": "5G NR BG1 Z=384 (K=8448, N=25344), flooding scaled min-sum, "
                                "10 fixed iterations, batch 4096",
                    "bg": 1, "Z": Z, "n_iter": N_ITER, "batch": BATCH, "ebn0_db": EBN0_DB,
-                   "cw_per_block": head["cw_per_block"]},
+                   "launch_shape": head["launch_shape"]},
         "latency_ms": head["latency_ms"],
         "fer": head["fer"],
         "ber": head["ber"],

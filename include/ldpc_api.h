@@ -106,10 +106,20 @@ int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float sca
  * reduce_precision<half>, src/setup_exp.cu:33-38, on the device. */
 int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out);
 
-/* Codewords per workgroup of the decode launch (0 restores the context's default).
+/* Codewords per workgroup of the decode launch (0 restores the context's default: the shape is
+ * then chosen per batch).  At BG1 Z=384 the value 2 selects the slim layout (two codewords per
+ * 12-wavefront workgroup, one workgroup per CU, the degree-1 APPs in a device buffer of the context),
+ * which the default choice takes for batches of more than one codeword per CU; decodes in that
+ * layout on one context run one after another on the GPU, whatever stream issues them (under
+ * stream capture that order is not recorded: graphs sharing one context must not run concurrently).
  * No reference counterpart: the reference fixes n_cw_per_mcw at build time
  * (setup_gpu_grid src/setup_exp.cu:131-166). */
 int ldpc_set_launch(ldpc_ctx* ctx, int cw_per_block);
+
+/* The launch shape a plain or learned decode of n_cw codewords on ctx uses now: codewords per
+ * workgroup and workgroups.  No reference counterpart (setup_gpu_grid src/setup_exp.cu:131-166
+ * computes the reference's fixed grid). */
+int ldpc_launch_shape(const ldpc_ctx* ctx, int n_cw, int* cw_per_block, int* workgroups);
 
 /* Repack per-codeword packed decisions ([n_cw][hd_bytes], as ldpc_decode_async writes them) into
  * one bit stream over the concatenated [n_cw][n_vn] columns: bit g of out (byte g/8, bit g%8) is
@@ -192,7 +202,9 @@ int ldpc_decode_learned_async(ldpc_ctx* ctx, void* stream, const float* llr, int
  * harnesses that forbid hipMalloc).  Same arguments, outputs and results as
  * ldpc_decode_learned_async; with LDPC_FLAG_FALLBACK_PLAIN the fallback's temporaries live in the
  * fixed per-GPU array (LDPC_NOALLOC_SCRATCH_BYTES) instead of the context's grow-only scratch, and
- * the call makes no hipMalloc, hipFree or event.  Returns LDPC_ENOMEM, enqueueing nothing, when
+ * the call makes no hipMalloc or hipFree and creates no event (a slim-layout decode at BG1 Z=384
+ * uses the degree-1 buffer and ordering event its context allocated when it was created; see
+ * ldpc_set_launch).  Returns LDPC_ENOMEM, enqueueing nothing, when
  * the batch needs more than that array (per codeword about 16 bytes + hd_bytes, plus n_vn floats
  * with app_out: at BG1 Z=384 up to about 10,200 codewords without app_out, 310 with it).  Every
  * call on one GPU shares that GPU's array, whatever context or stream issues it, and finds it

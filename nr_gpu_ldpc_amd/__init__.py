@@ -80,6 +80,7 @@ _sig("ldpc_ctx_destroy", ctypes.c_int, _P)
 _sig("ldpc_ctx_info", ctypes.c_int, _P, ctypes.POINTER(CodeInfo))
 _sig("ldpc_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
+_sig("ldpc_launch_shape", ctypes.c_int, _P, ctypes.c_int, _P, _P)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_decode_half_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
@@ -221,6 +222,12 @@ class Code:
     def set_launch(self, cw_per_block):
         _check(lib.ldpc_set_launch(self._h, int(cw_per_block)), "ldpc_set_launch")
         self.cw_per_block = cw_per_block or self.info["cw_per_block"]
+
+    def launch_shape(self, n_cw):
+        """(codewords per workgroup, workgroups) of a plain or learned decode of n_cw codewords."""
+        cpb, wg = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.ldpc_launch_shape(self._h, int(n_cw), ctypes.byref(cpb), ctypes.byref(wg)), "ldpc_launch_shape")
+        return cpb.value, wg.value
 
     def decode(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
         _check(lib.ldpc_decode_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,

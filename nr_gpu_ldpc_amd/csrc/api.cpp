@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -239,6 +240,18 @@ int ldpc_set_launch(ldpc_ctx* x, int cpb) {
     }
     x->cpb = cpb;
     x->cpb_set = true;
+    return LDPC_OK;
+}
+
+int ldpc_launch_shape(const ldpc_ctx* x, int n_cw, int* cw_per_block, int* workgroups) {
+    if (!x || n_cw < 1 || !cw_per_block || !workgroups) return LDPC_EINVAL;
+    if (x->device >= 0 && slim_chosen(x, n_cw)) {
+        *cw_per_block = decoder_slim_cpb(x->code.kind, x->code.Z);
+        *workgroups = std::min((n_cw + *cw_per_block - 1) / *cw_per_block, x->d1_slots);
+    } else {
+        *cw_per_block = x->cpb;
+        *workgroups = (n_cw + x->cpb - 1) / x->cpb;
+    }
     return LDPC_OK;
 }
 

@@ -49,6 +49,8 @@ int n_nd(int kind);
 int default_cpb(const CodeDesc& c);
 // Per-edge LDS offsets of the any-Z decoder for code c (row-major edge order, whole base graph).
 void build_edge_params(const CodeDesc& c, std::vector<EdgeParam>* out);
+// Whether a plain or learned decode of n_cw codewords on x runs the slim layout.
+bool slim_chosen(const ldpc_ctx* x, int n_cw);
 // Shared decode path; weights != null selects the learned decoder with [(n_iter+1)][nnz] weights.
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,

@@ -9,6 +9,11 @@
 
 namespace nrldpc {
 
+bool slim_chosen(const ldpc_ctx* x, int n_cw) {
+    const int slim_cpb = x->d_d1 ? decoder_slim_cpb(x->code.kind, x->code.Z) : 0;
+    return slim_cpb > 0 && (x->cpb_set ? x->cpb == slim_cpb : decoder_slim_pays(n_cw, x->d1_slots, slim_cpb, x->cpb));
+}
+
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,
                 const int* list_count, int* fail_out) {
@@ -38,10 +43,8 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     const hipStream_t s = static_cast<hipStream_t>(stream);
     // The slim layout: explicitly chosen (ldpc_set_launch with its codewords per workgroup), or
     // by batch size when the launch shape is left to the context.
-    const int slim_cpb = x->d_d1 ? decoder_slim_cpb(c.kind, c.Z) : 0;
-    const bool slim = slim_cpb > 0 && (x->cpb_set ? x->cpb == slim_cpb
-                                                  : decoder_slim_pays(n_cw, x->d1_slots, slim_cpb, x->cpb));
-    if (!slim) return decoder_launch(c.kind, weights != nullptr, a, s);
+    if (!slim_chosen(x, n_cw)) return decoder_launch(c.kind, weights != nullptr, a, s);
+    const int slim_cpb = decoder_slim_cpb(c.kind, c.Z);
     a.cpb = slim_cpb;
     a.lds_cw_bytes = (int)decoder_slim_lds_bytes_per_cw(c.kind, c.Z);
     a.d1_scratch = x->d_d1;

@@ -141,3 +141,19 @@ def test_slim_decodes_on_two_streams(nr, torch_gpu):
         hd = np.unpackbits(d_hd.cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
         assert np.array_equal(hd, want[0])
         assert np.array_equal(d_app.cpu().numpy().view(np.uint32), want[1].view(np.uint32))
+
+
+def test_launch_shape_reports_the_layout(nr, torch_gpu):
+    """The slim layout is chosen for batches of more than one codeword per CU (one workgroup per
+    CU), the default layout for smaller ones, and ldpc_set_launch fixes either."""
+    cus = torch_gpu.cuda.get_device_properties(0).multi_processor_count
+    code = nr.Code(1, 384)
+    assert code.launch_shape(8 * cus) == (SLIM, cus)
+    assert code.launch_shape(cus) == (DEFAULT, cus)
+    code.set_launch(SLIM)
+    assert code.launch_shape(3) == (SLIM, 2)
+    code.set_launch(DEFAULT)
+    assert code.launch_shape(8 * cus) == (DEFAULT, 8 * cus)
+    bg2 = nr.Code(2, 208)
+    cpb = bg2.info["cw_per_block"]
+    assert bg2.launch_shape(4096) == (cpb, (4096 + cpb - 1) // cpb)
