@@ -5,7 +5,8 @@
  * Device pointers are HIP device memory; `stream` is a hipStream_t passed as void*
  * (NULL = the default stream).  Every *_async call only enqueues work on `stream`.
  * A context may be used from several host threads as long as calls on it are not
- * concurrent with ldpc_ctx_destroy / ldpc_set_launch / ldpc_noms_set_weights.
+ * concurrent with ldpc_ctx_destroy / ldpc_set_launch / ldpc_set_launch_layout / ldpc_autotune /
+ * ldpc_noms_set_weights.
  *
  * Each entry point names the reference interface it replaces (paths relative to the
  * reference checkout).
@@ -106,20 +107,50 @@ int ldpc_llr_from_int8_async(void* stream, const int8_t* in, size_t n, float sca
  * reduce_precision<half>, src/setup_exp.cu:33-38, on the device. */
 int ldpc_llr_to_half_async(void* stream, const float* llr, size_t n, uint16_t* out);
 
-/* Codewords per workgroup of the decode launch (0 restores the context's default: the shape is
- * then chosen per batch).  At BG1 Z=384 the value 2 selects the slim layout (two codewords per
- * 12-wavefront workgroup, one workgroup per CU, the degree-1 APPs in a device buffer of the context),
- * which the default choice takes for batches of more than one codeword per CU; decodes in that
- * layout on one context run one after another on the GPU, whatever stream issues them (under
- * stream capture that order is not recorded: graphs sharing one context must not run concurrently).
+/* Launch shape of plain and learned decodes: an LDS layout and codewords per workgroup.
+ *   LDPC_LAYOUT_DUP   per codeword a read copy of the posteriors with every sub-column stored
+ *                     twice (the circulant read never wraps) and an accumulator, the degree-1
+ *                     (extension parity) posteriors in registers; up to 512 threads per workgroup,
+ *                     one workgroup per cw_per_block codewords
+ *   LDPC_LAYOUT_SLIM  a single read copy and the accumulator, the degree-1 posteriors in a device
+ *                     buffer of the context (one slot per CU); up to 768 threads (three
+ *                     wavefronts per SIMD), a persistent grid of one workgroup per CU
+ * Both give identical outputs.  LDPC_LAYOUT_AUTO (cw_per_block 0) chooses per batch the layout
+ * with the shorter modelled time: ceil(n_cw / codewords per round) rounds of a measured round time,
+ * each layout at its fastest codewords per workgroup, from the table built into the library
+ * (measured on MI355X) or from ldpc_autotune.  cw_per_block 0 with DUP or SLIM takes that layout's
+ * tuned count.  Decodes in the slim layout on one context run one after another on the GPU,
+ * whatever stream issues them (under stream capture that order is not recorded: graphs sharing one
+ * context must not run concurrently).  A set-up call: it may allocate the slim layout's buffer.
  * No reference counterpart: the reference fixes n_cw_per_mcw at build time
  * (setup_gpu_grid src/setup_exp.cu:131-166). */
+#define LDPC_LAYOUT_AUTO 0
+#define LDPC_LAYOUT_DUP 1
+#define LDPC_LAYOUT_SLIM 2
+int ldpc_set_launch_layout(ldpc_ctx* ctx, int layout, int cw_per_block);
+
+/* Codewords per workgroup of the decode launch: counts that fit the dup layout select it, larger
+ * ones the slim layout (at BG1 Z=384: 1 dup, 2 slim); 0 restores the per-batch choice
+ * (LDPC_LAYOUT_AUTO).  Same reference note as ldpc_set_launch_layout. */
 int ldpc_set_launch(ldpc_ctx* ctx, int cw_per_block);
 
-/* The launch shape a plain or learned decode of n_cw codewords on ctx uses now: codewords per
- * workgroup and workgroups.  No reference counterpart (setup_gpu_grid src/setup_exp.cu:131-166
- * computes the reference's fixed grid). */
+/* The launch shape a plain or learned decode of n_cw codewords on ctx uses now: layout (may be
+ * NULL), codewords per workgroup and workgroups.  No reference counterpart (setup_gpu_grid
+ * src/setup_exp.cu:131-166 computes the reference's fixed grid). */
+int ldpc_launch_layout(const ldpc_ctx* ctx, int n_cw, int* layout, int* cw_per_block, int* workgroups);
 int ldpc_launch_shape(const ldpc_ctx* ctx, int n_cw, int* cw_per_block, int* workgroups);
+
+/* Autotuner of the launch shape: times every candidate (layout, codewords per workgroup) on whole
+ * rounds of the device (every CU busy; n_iter fixed iterations, no early exit, random channel
+ * LLRs), installs the fastest of each layout as ctx's cost model and returns to the per-batch
+ * choice (LDPC_LAYOUT_AUTO).  Candidates: per layout each count that fills one more wavefront and
+ * the largest that fits.  Candidate k (k < max_candidates) is reported as cand_layout[k],
+ * cand_cpb[k], cand_cw_per_cu[k] (codewords per CU in a round) and cand_round_us[k] (microseconds
+ * per round, the median of three launches); any of them may be NULL.  A set-up call: it allocates
+ * up to 1 GiB of temporaries and synchronises the device.  No reference counterpart (the reference
+ * fixes the grid at build time, src/setup_exp.cu:131-166). */
+int ldpc_autotune(ldpc_ctx* ctx, int n_iter, int max_candidates, int* cand_layout, int* cand_cpb,
+                  int* cand_cw_per_cu, float* cand_round_us, int* n_candidates);
 
 /* Repack per-codeword packed decisions ([n_cw][hd_bytes], as ldpc_decode_async writes them) into
  * one bit stream over the concatenated [n_cw][n_vn] columns: bit g of out (byte g/8, bit g%8) is

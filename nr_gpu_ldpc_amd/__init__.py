@@ -25,6 +25,7 @@ LEARNED = 4
 NOALLOC_SCRATCH_BYTES = 32 << 20  # LDPC_NOALLOC_SCRATCH_BYTES of ldpc_api.h
 LLR_F32, LLR_I8, LLR_F16 = 0, 1, 2  # LDPC_LLR_* element types of mixed-batch LLRs
 CRC16, CRC24A, CRC24B = 0, 1, 2
+LAYOUT_AUTO, LAYOUT_DUP, LAYOUT_SLIM = 0, 1, 2  # LDPC_LAYOUT_* launch layouts
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -81,6 +82,9 @@ _sig("ldpc_ctx_info", ctypes.c_int, _P, ctypes.POINTER(CodeInfo))
 _sig("ldpc_decode_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_set_launch", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_launch_shape", ctypes.c_int, _P, ctypes.c_int, _P, _P)
+_sig("ldpc_set_launch_layout", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int)
+_sig("ldpc_launch_layout", ctypes.c_int, _P, ctypes.c_int, _P, _P, _P)
+_sig("ldpc_autotune", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_decode_half_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
@@ -228,6 +232,27 @@ class Code:
         cpb, wg = ctypes.c_int(0), ctypes.c_int(0)
         _check(lib.ldpc_launch_shape(self._h, int(n_cw), ctypes.byref(cpb), ctypes.byref(wg)), "ldpc_launch_shape")
         return cpb.value, wg.value
+
+    def set_launch_layout(self, layout, cw_per_block=0):
+        """LAYOUT_AUTO (per batch), LAYOUT_DUP or LAYOUT_SLIM with codewords per workgroup (0: tuned)."""
+        _check(lib.ldpc_set_launch_layout(self._h, int(layout), int(cw_per_block)), "ldpc_set_launch_layout")
+
+    def launch_layout(self, n_cw):
+        """(layout, codewords per workgroup, workgroups) of a plain or learned decode of n_cw codewords."""
+        lay, cpb, wg = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib.ldpc_launch_layout(self._h, int(n_cw), ctypes.byref(lay), ctypes.byref(cpb), ctypes.byref(wg)),
+               "ldpc_launch_layout")
+        return lay.value, cpb.value, wg.value
+
+    def autotune(self, n_iter=10, max_candidates=64):
+        """Time every launch shape and install the fastest of each layout; returns the candidates as
+        dicts (layout, cpb, cw_per_cu, round_us)."""
+        n = int(max_candidates)
+        lay, cpb, per = (ctypes.c_int * n)(), (ctypes.c_int * n)(), (ctypes.c_int * n)()
+        rus, cnt = (ctypes.c_float * n)(), ctypes.c_int(0)
+        _check(lib.ldpc_autotune(self._h, int(n_iter), n, lay, cpb, per, rus, ctypes.byref(cnt)), "ldpc_autotune")
+        return [{"layout": lay[k], "cpb": cpb[k], "cw_per_cu": per[k], "round_us": rus[k]}
+                for k in range(min(n, cnt.value))]
 
     def decode(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
         _check(lib.ldpc_decode_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,

@@ -113,16 +113,19 @@ int ctx_build_device(ldpc_ctx* x) {
     if ((st = upload(x->prog.step_start, &x->d_step_start))) return st;
     if ((st = upload(x->prog.term_col, &x->d_term_col))) return st;
     if ((st = upload(x->prog.term_shift, &x->d_term_shift))) return st;
-    // degree-1 APP slots of the slim layout: one workgroup slot per CU
-    if (decoder_slim_cpb(c.kind, c.Z) > 0) {
-        int cus = 0;
-        NRLDPC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, x->device));
-        if (cus > 0) {
-            NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&x->d_d1), (size_t)cus * decoder_slim_slot_bytes(c.kind, c.Z)));
-            NRLDPC_HIP(hipEventCreateWithFlags(&x->d1_done, hipEventDisableTiming));
-            x->d1_slots = cus;
-        }
+    // launch shape: the built-in tuning table of (kind, Z), else the dup layout at the default
+    // codewords per workgroup; the slim layout's degree-1 slots when the table uses it
+    NRLDPC_HIP(hipDeviceGetAttribute(&x->cus, hipDeviceAttributeMultiprocessorCount, x->device));
+    x->m_dup = LayoutModel{};
+    x->m_slim = LayoutModel{};
+    x->m_dup.cpb = x->cpb_default;
+    if (const LayoutTuning* t = builtin_layout_tuning(c.kind, c.Z)) {
+        if (t->dup.cpb > 0 && t->dup.cpb <= decoder_dup_cpb_max(c.kind, c.Z)) x->m_dup = t->dup;
+        if (t->slim.cpb > 0 && t->slim.cpb <= decoder_slim_cpb_max(c.kind, c.Z)) x->m_slim = t->slim;
     }
+    x->cpb_default = x->m_dup.cpb;
+    x->cpb = x->cpb_default;
+    if (x->m_slim.cpb > 0 && (st = ensure_slim(x, x->m_slim.cpb))) return st;
     return LDPC_OK;
 }
 
@@ -229,30 +232,75 @@ int ldpc_ctx_info(const ldpc_ctx* x, ldpc_code_info* info) {
     return LDPC_OK;
 }
 
+int ldpc_set_launch_layout(ldpc_ctx* x, int layout, int cpb) {
+    if (!x || cpb < 0) return LDPC_EINVAL;
+    const int kind = x->code.kind, Z = x->code.Z;
+    if (layout == LDPC_LAYOUT_AUTO) {
+        if (cpb != 0) { set_last_error("the automatic layout takes cw_per_block 0"); return LDPC_EINVAL; }
+        x->cpb = x->cpb_default;
+        if (x->m_slim.cpb > 0 && x->device >= 0) {
+            const int st = ensure_slim(x, x->m_slim.cpb);
+            if (st) return st;
+        }
+        x->layout_set = 0;
+        return LDPC_OK;
+    }
+    if (layout == LDPC_LAYOUT_DUP) {
+        const int c = cpb ? cpb : x->cpb_default;
+        if (c > decoder_dup_cpb_max(kind, Z)) {
+            set_last_error("cw_per_block exceeds the workgroup size or LDS limit of the dup layout");
+            return LDPC_EINVAL;
+        }
+        x->cpb = c;
+        x->layout_set = kLayoutDup;
+        return LDPC_OK;
+    }
+    if (layout == LDPC_LAYOUT_SLIM) {
+        const int c = cpb ? cpb : (x->m_slim.cpb > 0 ? x->m_slim.cpb : decoder_slim_cpb_max(kind, Z));
+        const int st = ensure_slim(x, c);
+        if (st) return st;
+        x->layout_set = kLayoutSlim;
+        return LDPC_OK;
+    }
+    set_last_error("unknown layout");
+    return LDPC_EINVAL;
+}
+
 int ldpc_set_launch(ldpc_ctx* x, int cpb) {
     if (!x || cpb < 0) return LDPC_EINVAL;
-    if (cpb == 0) { x->cpb = x->cpb_default; x->cpb_set = false; return LDPC_OK; }
-    const size_t lds = decoder_lds_bytes_per_cw(x->code.kind, x->code.Z);
-    const bool slim = x->d_d1 != nullptr && cpb == decoder_slim_cpb(x->code.kind, x->code.Z);
-    if (!slim && (cpb * x->code.Z > kMaxBlock || cpb * lds + 16 > (size_t)max_lds_bytes())) {
-        set_last_error("cw_per_block exceeds the workgroup size or LDS limit");
-        return LDPC_EINVAL;
+    if (cpb == 0) return ldpc_set_launch_layout(x, LDPC_LAYOUT_AUTO, 0);
+    if (cpb <= decoder_dup_cpb_max(x->code.kind, x->code.Z)) return ldpc_set_launch_layout(x, LDPC_LAYOUT_DUP, cpb);
+    if (x->device >= 0 && cpb <= decoder_slim_cpb_max(x->code.kind, x->code.Z))
+        return ldpc_set_launch_layout(x, LDPC_LAYOUT_SLIM, cpb);
+    set_last_error("cw_per_block exceeds the workgroup size or LDS limit");
+    return LDPC_EINVAL;
+}
+
+int ldpc_launch_layout(const ldpc_ctx* x, int n_cw, int* layout, int* cw_per_block, int* workgroups) {
+    if (!x || n_cw < 1) return LDPC_EINVAL;
+    int lay = LDPC_LAYOUT_DUP, c = x->cpb, wg = (n_cw + x->cpb - 1) / x->cpb;
+    if (x->device >= 0 && slim_chosen(x, n_cw)) {
+        lay = LDPC_LAYOUT_SLIM;
+        c = x->slim_cpb;
+        wg = std::min((n_cw + c - 1) / c, x->cus);
     }
-    x->cpb = cpb;
-    x->cpb_set = true;
+    if (layout) *layout = lay;
+    if (cw_per_block) *cw_per_block = c;
+    if (workgroups) *workgroups = wg;
     return LDPC_OK;
 }
 
 int ldpc_launch_shape(const ldpc_ctx* x, int n_cw, int* cw_per_block, int* workgroups) {
     if (!x || n_cw < 1 || !cw_per_block || !workgroups) return LDPC_EINVAL;
-    if (x->device >= 0 && slim_chosen(x, n_cw)) {
-        *cw_per_block = decoder_slim_cpb(x->code.kind, x->code.Z);
-        *workgroups = std::min((n_cw + *cw_per_block - 1) / *cw_per_block, x->d1_slots);
-    } else {
-        *cw_per_block = x->cpb;
-        *workgroups = (n_cw + x->cpb - 1) / x->cpb;
-    }
-    return LDPC_OK;
+    return ldpc_launch_layout(x, n_cw, nullptr, cw_per_block, workgroups);
+}
+
+int ldpc_autotune(ldpc_ctx* x, int n_iter, int max_candidates, int* cand_layout, int* cand_cpb, int* cand_cw_per_cu,
+                  float* cand_round_us, int* n_candidates) {
+    if (!x || n_iter < 1 || max_candidates < 0) return LDPC_EINVAL;
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    return autotune_layouts(x, n_iter, max_candidates, cand_layout, cand_cpb, cand_cw_per_cu, cand_round_us,
+                            n_candidates);
 }
 
 int ldpc_decode_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,

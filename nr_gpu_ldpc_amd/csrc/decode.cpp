@@ -10,8 +10,31 @@
 namespace nrldpc {
 
 bool slim_chosen(const ldpc_ctx* x, int n_cw) {
-    const int slim_cpb = x->d_d1 ? decoder_slim_cpb(x->code.kind, x->code.Z) : 0;
-    return slim_cpb > 0 && (x->cpb_set ? x->cpb == slim_cpb : decoder_slim_pays(n_cw, x->d1_slots, slim_cpb, x->cpb));
+    if (x->slim_cpb <= 0 || x->d1_done == nullptr) return false;  // slim resources not set up
+    if (x->layout_set) return x->layout_set == kLayoutSlim;
+    return layout_for_batch(x->m_dup, x->m_slim, n_cw, x->cus) == kLayoutSlim;
+}
+
+int ensure_slim(ldpc_ctx* x, int cpb) {
+    if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
+    if (cpb < 1 || cpb > decoder_slim_cpb_max(x->code.kind, x->code.Z) || x->cus < 1) {
+        set_last_error("slim layout: codewords per workgroup beyond the workgroup size or LDS limit");
+        return LDPC_EINVAL;
+    }
+    NRLDPC_HIP(hipSetDevice(x->device));
+    if (!x->d1_done) NRLDPC_HIP(hipEventCreateWithFlags(&x->d1_done, hipEventDisableTiming));
+    const size_t slot = decoder_slim_slot_bytes(x->code.kind, x->code.Z, cpb);
+    if (slot > 0 && (x->d_d1 == nullptr || cpb > x->d1_cpb)) {
+        std::lock_guard<std::mutex> lk(x->d1_mu);
+        NRLDPC_HIP(hipEventSynchronize(x->d1_done));  // launches on the old slots have finished
+        if (x->d_d1) NRLDPC_HIP(hipFree(x->d_d1));
+        x->d_d1 = nullptr;
+        x->d1_cpb = 0;
+        NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&x->d_d1), (size_t)x->cus * slot));
+        x->d1_cpb = cpb;
+    }
+    x->slim_cpb = cpb;
+    return LDPC_OK;
 }
 
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
@@ -44,11 +67,11 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     // The slim layout: explicitly chosen (ldpc_set_launch with its codewords per workgroup), or
     // by batch size when the launch shape is left to the context.
     if (!slim_chosen(x, n_cw)) return decoder_launch(c.kind, weights != nullptr, a, s);
-    const int slim_cpb = decoder_slim_cpb(c.kind, c.Z);
+    const int slim_cpb = x->slim_cpb;
     a.cpb = slim_cpb;
     a.lds_cw_bytes = (int)decoder_slim_lds_bytes_per_cw(c.kind, c.Z);
     a.d1_scratch = x->d_d1;
-    const int grid = std::min((n_cw + slim_cpb - 1) / slim_cpb, x->d1_slots);
+    const int grid = std::min((n_cw + slim_cpb - 1) / slim_cpb, x->cus);
     // Launches on one context share its slots: each waits for the previous one (on any stream).
     // Under stream capture the wait is not recorded: the captured graph orders its own nodes,
     // and graphs that share one context must not run concurrently.

@@ -50,7 +50,7 @@ struct DecodeArgs {
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
     const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
-    float* d1_scratch;         // two-workgroups-per-CU kernel: APP of the degree-1 columns,
+    float* d1_scratch;         // slim-layout kernel: APP of the degree-1 columns,
                                // [gridDim.x][cpb][ND1][Z] floats (one slot per resident workgroup)
 };
 
@@ -111,19 +111,44 @@ int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, si
 // Resident workgroups per CU of the mixed kernel at this shape (0 on error).
 int mixed_occupancy(bool learned, int threads, size_t lds);
 
+// LDS tilings of the fused FP32 decoder (plain and learned), the autotuner's layout axis:
+//   kLayoutDup   read copy [NND][2Z] (sub-columns stored twice: the circulant read never wraps)
+//                and accumulator [NND][Z] per codeword, degree-1 APPs in VGPRs; workgroups of
+//                up to kMaxBlock threads, one per cpb codewords (k_decode)
+//   kLayoutSlim  read copy [NND][Z] (the read selects between two bases like the accumulator
+//                update) and accumulator [NND][Z], degree-1 APPs in the workgroup's slot of a
+//                device buffer; up to kSlimMaxBlock threads at three wavefronts per SIMD, a
+//                persistent grid of one workgroup per CU (k_decode_slim)
+// Both run the same arithmetic in the same order: their outputs are identical.
+constexpr int kLayoutDup = 1, kLayoutSlim = 2;
+
 size_t decoder_lds_bytes_per_cw(int kind, int Z);
-// The slim layout (k_decode_slim): a single LDS read copy and the degree-1 APPs in a device
-// slot per resident workgroup, so that a workgroup of decoder_slim_cpb codewords (12 wavefronts
-// at BG1 Z=384) fits one CU at three wavefronts per SIMD.  Compiled for BG1 Z=384 only
-// (decoder_slim_cpb returns 0 elsewhere).
-int decoder_slim_cpb(int kind, int Z);
 size_t decoder_slim_lds_bytes_per_cw(int kind, int Z);
-size_t decoder_slim_slot_bytes(int kind, int Z);  // a.d1_scratch bytes per workgroup slot
-// Persistent launch of `grid` workgroups (a.d1_scratch holds grid slots).
+// Largest codewords per workgroup of each layout (threads and LDS); 0 if none fits.
+int decoder_dup_cpb_max(int kind, int Z);
+int decoder_slim_cpb_max(int kind, int Z);
+size_t decoder_slim_slot_bytes(int kind, int Z, int cpb);  // a.d1_scratch bytes per workgroup slot
+// Persistent slim launch of `grid` workgroups of a.cpb codewords (a.d1_scratch holds grid slots).
 int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s);
-// Whether the slim launch is expected to finish n_cw codewords sooner than the default layout on
-// a device with `cus` compute units (per-round costs measured on MI355X).
-bool decoder_slim_pays(int n_cw, int cus, int slim_cpb, int default_cpb);
+// Resident workgroups per CU of the dup-layout kernel at this shape (0 on error).
+int decoder_dup_occupancy(int kind, int Z, bool learned, int threads, size_t lds);
+
+// Cost model of one layout for one (kind, Z): a round is every CU decoding cw_per_cu codewords
+// (cpb per workgroup) and takes round_us; n codewords take ceil(n / (cw_per_cu * CUs)) rounds.
+struct LayoutModel {
+    int cpb = 0;          // 0: layout not in use
+    int cw_per_cu = 0;
+    float round_us = 0.f;
+};
+struct LayoutTuning {
+    int kind, Z;
+    LayoutModel dup, slim;
+};
+// The measured table compiled into the library (decode_tuning.inc), or null for an untuned code.
+const LayoutTuning* builtin_layout_tuning(int kind, int Z);
+// The layout with the shorter modelled time for n_cw codewords on `cus` CUs (kLayoutDup on ties
+// or when slim is not in use).
+int layout_for_batch(const LayoutModel& dup, const LayoutModel& slim, int n_cw, int cus);
 int max_lds_bytes();  // per-workgroup dynamic LDS limit of the current device
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s);
 // The reference's HALF_PRC schedule (binary16 LLRs in a.llr_half).

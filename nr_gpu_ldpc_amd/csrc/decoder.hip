@@ -1,6 +1,8 @@
 // decoder.hip -- host-side launch of the fused decoder kernels (decoder_kernel.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "code.h"
 #include "decoder.h"
 #include "status.h"
@@ -45,7 +47,11 @@ size_t decoder_lds_bytes_per_cw(int kind, int Z) {
     return ((size_t)3 * nnd * 4 * Z + 16 + 15) & ~(size_t)15;
 }
 
-int decoder_slim_cpb(int kind, int Z) { return kind == kBG1 && Z == kBG1Special ? 2 : 0; }
+static int n_nd1(int kind) {
+    if (kind == kBG1) return Structure<kBG1>::ND1;
+    if (kind == kBG2) return Structure<kBG2>::ND1;
+    return Structure<kWifi>::ND1;
+}
 
 size_t decoder_slim_lds_bytes_per_cw(int kind, int Z) {
     int nnd = 0;
@@ -55,36 +61,73 @@ size_t decoder_slim_lds_bytes_per_cw(int kind, int Z) {
     return ((size_t)2 * nnd * 4 * Z + 16 + 15) & ~(size_t)15;
 }
 
-size_t decoder_slim_slot_bytes(int kind, int Z) {
-    int nd1 = 0;
-    if (kind == kBG1) nd1 = Structure<kBG1>::ND1;
-    else if (kind == kBG2) nd1 = Structure<kBG2>::ND1;
-    return (size_t)decoder_slim_cpb(kind, Z) * nd1 * Z * sizeof(float);
+// Codewords per workgroup that fit `threads` and the LDS (+16 bytes: the workgroup flag word).
+static int cpb_max_for(int Z, int threads, size_t lds_cw) {
+    const size_t lim = (size_t)max_lds_bytes();
+    if (Z < 1 || Z > threads || lds_cw + 16 > lim) return 0;
+    return (int)std::min<size_t>((size_t)(threads / Z), (lim - 16) / lds_cw);
 }
 
-// One round of the slim kernel (every CU decoding slim_cpb codewords) takes this many times as
-// long as one round of the default layout (every CU decoding default_cpb codewords): BG1 Z=384
-// T=10, 4096 codewords, 3.72 ms in 8 rounds against 4.29 ms in 16 on MI355X.
-constexpr double kSlimRoundCost = 1.73;
+int decoder_dup_cpb_max(int kind, int Z) { return cpb_max_for(Z, kMaxBlock, decoder_lds_bytes_per_cw(kind, Z)); }
+int decoder_slim_cpb_max(int kind, int Z) { return cpb_max_for(Z, kSlimMaxBlock, decoder_slim_lds_bytes_per_cw(kind, Z)); }
 
-bool decoder_slim_pays(int n_cw, int cus, int slim_cpb, int default_cpb) {
-    if (slim_cpb <= 0 || cus <= 0 || n_cw <= 0) return false;
-    const long long per_round_d = (long long)cus * default_cpb, per_round_s = (long long)cus * slim_cpb;
-    const long long rounds_d = (n_cw + per_round_d - 1) / per_round_d;
-    const long long rounds_s = (n_cw + per_round_s - 1) / per_round_s;
-    return (double)rounds_s * kSlimRoundCost < (double)rounds_d;
+size_t decoder_slim_slot_bytes(int kind, int Z, int cpb) {
+    return (size_t)cpb * n_nd1(kind) * Z * sizeof(float);
+}
+
+int layout_for_batch(const LayoutModel& dup, const LayoutModel& slim, int n_cw, int cus) {
+    if (slim.cpb <= 0 || slim.cw_per_cu <= 0 || cus <= 0 || n_cw <= 0) return kLayoutDup;
+    if (dup.cpb <= 0 || dup.cw_per_cu <= 0) return kLayoutSlim;
+    const long long per_d = (long long)cus * dup.cw_per_cu, per_s = (long long)cus * slim.cw_per_cu;
+    const double t_d = (double)((n_cw + per_d - 1) / per_d) * dup.round_us;
+    const double t_s = (double)((n_cw + per_s - 1) / per_s) * slim.round_us;
+    return t_s < t_d ? kLayoutSlim : kLayoutDup;
+}
+
+// Instantiated in decoder_slim_{bg1,bg2,qc}.hip and decoder_{bg1,bg2,qc}.hip.
+template <int KIND, int ZC, bool LEARNED>
+int occupancy_decode(int threads, size_t lds);
+
+int decoder_dup_occupancy(int kind, int Z, bool learned, int threads, size_t lds) {
+    if (kind == kBG1) {
+        if (Z == kBG1Special) return learned ? occupancy_decode<kBG1, kBG1Special, true>(threads, lds)
+                                             : occupancy_decode<kBG1, kBG1Special, false>(threads, lds);
+        return learned ? occupancy_decode<kBG1, 0, true>(threads, lds) : occupancy_decode<kBG1, 0, false>(threads, lds);
+    }
+    if (kind == kBG2) {
+        if (Z == kBG2Special) return learned ? occupancy_decode<kBG2, kBG2Special, true>(threads, lds)
+                                             : occupancy_decode<kBG2, kBG2Special, false>(threads, lds);
+        return learned ? occupancy_decode<kBG2, 0, true>(threads, lds) : occupancy_decode<kBG2, 0, false>(threads, lds);
+    }
+    if (Z == kQCSpecial) return learned ? occupancy_decode<kWifi, kQCSpecial, true>(threads, lds)
+                                        : occupancy_decode<kWifi, kQCSpecial, false>(threads, lds);
+    return learned ? occupancy_decode<kWifi, 0, true>(threads, lds) : occupancy_decode<kWifi, 0, false>(threads, lds);
 }
 
 int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s) {
     const int threads = a.cpb * a.Z;
     const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;  // + workgroup flag slot
-    if (a.cpb != decoder_slim_cpb(kind, a.Z) || threads > kSlimMaxBlock || lds > (size_t)max_lds_bytes() ||
-        grid < 1 || !a.d1_scratch) {
+    if (a.cpb < 1 || a.cpb > decoder_slim_cpb_max(kind, a.Z) || threads > kSlimMaxBlock ||
+        lds > (size_t)max_lds_bytes() || grid < 1 || (n_nd1(kind) > 0 && !a.d1_scratch)) {
         set_last_error("slim decoder launch: unsupported shape");
         return LDPC_EINVAL;
     }
-    return learned ? launch_decode_slim<kBG1, kBG1Special, true>(a, grid, threads, lds, s)
-                   : launch_decode_slim<kBG1, kBG1Special, false>(a, grid, threads, lds, s);
+    if (kind == kBG1) {
+        if (a.Z == kBG1Special) return learned ? launch_decode_slim<kBG1, kBG1Special, true>(a, grid, threads, lds, s)
+                                               : launch_decode_slim<kBG1, kBG1Special, false>(a, grid, threads, lds, s);
+        return learned ? launch_decode_slim<kBG1, 0, true>(a, grid, threads, lds, s)
+                       : launch_decode_slim<kBG1, 0, false>(a, grid, threads, lds, s);
+    }
+    if (kind == kBG2) {
+        if (a.Z == kBG2Special) return learned ? launch_decode_slim<kBG2, kBG2Special, true>(a, grid, threads, lds, s)
+                                               : launch_decode_slim<kBG2, kBG2Special, false>(a, grid, threads, lds, s);
+        return learned ? launch_decode_slim<kBG2, 0, true>(a, grid, threads, lds, s)
+                       : launch_decode_slim<kBG2, 0, false>(a, grid, threads, lds, s);
+    }
+    if (a.Z == kQCSpecial) return learned ? launch_decode_slim<kWifi, kQCSpecial, true>(a, grid, threads, lds, s)
+                                          : launch_decode_slim<kWifi, kQCSpecial, false>(a, grid, threads, lds, s);
+    return learned ? launch_decode_slim<kWifi, 0, true>(a, grid, threads, lds, s)
+                   : launch_decode_slim<kWifi, 0, false>(a, grid, threads, lds, s);
 }
 
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s) {

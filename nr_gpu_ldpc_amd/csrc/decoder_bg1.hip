@@ -11,16 +11,15 @@ int launch_decode(const DecodeArgs& a, int blocks, int threads, size_t lds, hipS
     return LDPC_OK;
 }
 
+template <int KIND, int ZC, bool LEARNED>
+int occupancy_decode(int threads, size_t lds) {
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode<KIND, ZC, LEARNED>, threads, lds) == hipSuccess ? n : 0;
+}
+
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_decode<KIND, ZC, false, false, true>), dim3(blocks), dim3(threads), lds, s, a);
-    NRLDPC_HIP(hipGetLastError());
-    return LDPC_OK;
-}
-
-template <int KIND, int ZC, bool LEARNED>
-int launch_decode_slim(const DecodeArgs& a, int grid, int threads, size_t lds, hipStream_t s) {
-    hipLaunchKernelGGL((k_decode_slim<KIND, ZC, LEARNED>), dim3(grid), dim3(threads), lds, s, a);
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;
 }
@@ -36,9 +35,10 @@ template int launch_decode<kBG1, 0, false>(const DecodeArgs&, int, int, size_t, 
 template int launch_decode<kBG1, 0, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG1, 384, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG1, 384, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
-
-template int launch_decode_slim<kBG1, 384, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
-template int launch_decode_slim<kBG1, 384, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int occupancy_decode<kBG1, 0, false>(int, size_t);
+template int occupancy_decode<kBG1, 0, true>(int, size_t);
+template int occupancy_decode<kBG1, 384, false>(int, size_t);
+template int occupancy_decode<kBG1, 384, true>(int, size_t);
 
 template int launch_decode_train<kBG1>(const DecodeArgs&, int, int, size_t, hipStream_t);
 

@@ -11,6 +11,12 @@ int launch_decode(const DecodeArgs& a, int blocks, int threads, size_t lds, hipS
     return LDPC_OK;
 }
 
+template <int KIND, int ZC, bool LEARNED>
+int occupancy_decode(int threads, size_t lds) {
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode<KIND, ZC, LEARNED>, threads, lds) == hipSuccess ? n : 0;
+}
+
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s) {
     hipLaunchKernelGGL((k_decode<KIND, ZC, false, false, true>), dim3(blocks), dim3(threads), lds, s, a);
@@ -29,6 +35,10 @@ template int launch_decode<kBG2, 0, false>(const DecodeArgs&, int, int, size_t, 
 template int launch_decode<kBG2, 0, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG2, 208, false>(const DecodeArgs&, int, int, size_t, hipStream_t);
 template int launch_decode<kBG2, 208, true>(const DecodeArgs&, int, int, size_t, hipStream_t);
+template int occupancy_decode<kBG2, 0, false>(int, size_t);
+template int occupancy_decode<kBG2, 0, true>(int, size_t);
+template int occupancy_decode<kBG2, 208, false>(int, size_t);
+template int occupancy_decode<kBG2, 208, true>(int, size_t);
 
 template int launch_decode_train<kBG2>(const DecodeArgs&, int, int, size_t, hipStream_t);
 

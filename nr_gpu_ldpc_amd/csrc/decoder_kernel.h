@@ -48,6 +48,12 @@ __device__ __forceinline__ void static_for(F&& f) {
 __device__ __forceinline__ float as_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
 
+// Rows of at most this many edges keep each edge's wrap-selected base from the pass-1 read to the
+// VN update in the slim layout; larger rows recompute it.  Measured at BG1 Z=384, T=10, 4096
+// codewords: 10 (198 of the 266 wrapping edges) 3.27 ms per launch, 19 (every row) 3.52 ms (VGPR
+// spills 56 -> 94), none 3.29 ms.
+constexpr int kSlimShareDeg = 10;
+
 extern __shared__ __attribute__((aligned(16))) unsigned char g_smem[];
 
 // LDS accesses through address-space-3 pointers made from 32-bit byte addresses: the
@@ -186,6 +192,46 @@ __device__ __forceinline__ uint32_t acc_addr(int r, uint32_t rHi, uint32_t rAcc,
     }
 }
 
+// Slim layout with compiled-in offsets: the accumulator lies NND*Z*4 bytes above the read copy,
+// so one circulant wrap select per edge serves both of its accesses (pass-1 read at sel + rd(),
+// VN update at sel + ac()).  Edges whose accumulator offset would exceed the 16-bit LDS
+// immediate take bases raised by `shift`.
+template <int KIND, int ZC>
+struct SlimImm {
+    static constexpr uint32_t accd = (uint32_t)Structure<KIND>::NND * ZC * 4u;
+    static constexpr uint32_t max_off = (uint32_t)(Structure<KIND>::NND * ZC - 1) * 4u;
+    static constexpr uint32_t shift = max_off + accd > 0xFFFCu ? max_off + accd - 0xFFFCu : 0u;
+    template <int E, int J>
+    static constexpr uint32_t off() { return (uint32_t)(J * ZC + edge_vm<KIND, ZC>(E)) * 4u; }
+    template <int E, int J>
+    static constexpr bool hi() { return off<E, J>() + accd > 0xFFFCu; }
+    template <int E, int J>
+    static constexpr uint32_t rd() {
+        static_assert(!hi<E, J>() || off<E, J>() >= shift, "shifted read offset below zero");
+        return hi<E, J>() ? off<E, J>() - shift : off<E, J>();
+    }
+    template <int E, int J>
+    static constexpr uint32_t ac() { return rd<E, J>() + accd; }
+};
+
+// The base of edge E's circulant access: b, or bw for lanes whose (r + V) wraps (ZC > 0).
+template <int ZC, int KIND, int E>
+__device__ __forceinline__ uint32_t wrap_sel(int r, uint32_t b, uint32_t bw) {
+    constexpr int vm = edge_vm<KIND, ZC>(E);
+    if constexpr (vm == 0) {
+        return b;
+    } else if constexpr (ZC - vm > 64) {
+        uint32_t res;  // literal-operand compare, as in acc_addr
+        asm("v_cmp_gt_i32_e32 vcc, %1, %2\n\tv_cndmask_b32_e32 %0, %3, %4, vcc"
+            : "=v"(res)
+            : "i"(ZC - vm), "v"(r), "v"(bw), "v"(b)
+            : "vcc");
+        return res;
+    } else {
+        return r >= ZC - vm ? bw : b;
+    }
+}
+
 // One workgroup's share of a decode: codewords blk*cpb .. blk*cpb+cpb-1.
 // ZC > 0: specialised for one lifting size, every circulant offset is an immediate.
 // ZC == 0: any lifting size, offsets come from the per-edge table (scalar loads).
@@ -267,6 +313,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     uint32_t rAccW = rAcc - Z4;                    // same, for lanes whose (r+V) wraps
     uint32_t rRdHi = SLIM ? rRd - Z4 : rRd + 0x10000u;  // SLIM: read copy for wrapping lanes;
                                                    // else a second base (16-bit immediates)
+    // SLIM with compiled-in offsets: the same two bases raised for the high columns (SlimImm)
+    constexpr uint32_t kShift = (SLIM && ZC > 0) ? SlimImm<KIND, (ZC > 0 ? ZC : 1)>::shift : 0u;
+    uint32_t rRdS = rRd + kShift, rRdHiS = rRdHi + kShift;
     const uint32_t rHi = ((uint32_t)r << 16) | 0xFFFFu;  // compared with EdgeParam::acc
     const uint32_t flagAddr = base + (RDN + 1u) * NND * Z4;
     // SLIM: this lane's degree-1 APPs, slot s at byte d1v + s * 4Z of the workgroup's scratch
@@ -373,7 +422,9 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     if (ap_out) ap_out[j * Z + c] = x;
                 }
             });
-            if (er < 8) lds_stb(eb + nv + er, 0);
+            // zero the staging bytes up to the next multiple of 8 columns (the packed padding
+            // bits), stepping by Z: fewer than 8 lanes per codeword when Z < 8
+            for (int p = er; p < 8; p += Z) lds_stb(eb + nv + p, 0);
         }
         __syncthreads();
         if (mine) {
@@ -401,6 +452,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
         // out of the iteration loop into long-lived VGPRs.
         asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW), "+v"(rRdHi));
+        if constexpr (SLIM && ZC > 0) asm volatile("" : "+v"(rRdS), "+v"(rRdHiS));
         CW2* w_old = LEARNED ? wt + (size_t)t * a.nnz : nullptr;
         CW2* w_new = LEARNED ? wt + (size_t)(t + 1) * a.nnz : nullptr;
         static_for<0, ROWS>([&](auto IC) {
@@ -424,6 +476,10 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             if constexpr (SLIM && row_d1_slot<KIND>(i) >= 0) d1x = d1_ld(row_d1_slot<KIND>(i));
             float dtv[D];
             float xd1[D];  // SLIM: APP_old of the row's degree-1 edges
+            // SLIM, compiled-in offsets: the wrap-selected base of each edge, from the pass-1 read
+            // to the VN update
+            constexpr bool SHARE = SLIM && ZC > 0 && D <= kSlimShareDeg;
+            uint32_t selv[D];
             if (!idle) {
                 float q[D], ro[D];
                 float m1 = 1000.0f, m2 = 1000.0f;
@@ -440,8 +496,17 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     constexpr int j = S::T.ecol[e];
                     float x;
                     if constexpr (j < NND) {
-                        if constexpr (SLIM) x = lds_ld(acc_addr<ZC, KIND, e, j>(rr, rrHi, rRd, rRdHi, et));
-                        else x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                        if constexpr (SHARE) {
+                            using SI = SlimImm<KIND, (ZC > 0 ? ZC : 1)>;
+                            constexpr bool hi = SI::template hi<e, j>();
+                            const uint32_t sb = wrap_sel<ZC, KIND, e>(rr, hi ? rRdS : rRd, hi ? rRdHiS : rRdHi);
+                            selv[k] = sb;
+                            x = lds_ld(sb + SI::template rd<e, j>());
+                        } else if constexpr (SLIM) {
+                            x = lds_ld(acc_addr<ZC, KIND, e, j>(rr, rrHi, rRd, rRdHi, et));
+                        } else {
+                            x = read_copy<ZC, KIND, e, j>(rRd, rRdHi, et);
+                        }
                     } else if constexpr (SLIM) {
                         if constexpr (row_d1_slot<KIND>(i) == S::T.d1_slot[j]) x = d1x;
                         else x = d1_ld(S::T.d1_slot[j]);
@@ -536,7 +601,12 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     constexpr int e = E0 + k;
                     constexpr int j = S::T.ecol[e];
                     if constexpr (j < NND) {
-                        addr[KC] = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                        if constexpr (SHARE) {
+                            using SI = SlimImm<KIND, (ZC > 0 ? ZC : 1)>;
+                            addr[KC] = selv[k] + SI::template ac<e, j>();
+                        } else {
+                            addr[KC] = acc_addr<ZC, KIND, e, j>(rr, rrHi, rAcc, rAccW, et);
+                        }
                         cur[KC] = lds_ld(addr[KC]);
                     }
                 });

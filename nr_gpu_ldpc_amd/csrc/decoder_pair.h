@@ -146,6 +146,13 @@ struct PairRegs {
     }
     static constexpr Tab T = make();
     static constexpr int NRW = (T.n + 1) / 2;
+    // Whether the other half of edge e's register word belongs to the neighbouring edge of the
+    // same row (k: e's position in a row of degree d): then both halves are written together.
+    static constexpr bool partner_in_row(int e, int k, int d) {
+        const int n = T.idx[e];
+        if (n & 1) return k > 0 && T.idx[e - 1] == n - 1;
+        return k + 1 < d && T.idx[e + 1] == n + 1;
+    }
 };
 
 // Byte offset (relative to a region base) of edge E's sub-column in a doubled [NND][2Z] array.
@@ -322,8 +329,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                     if constexpr (j < NND) {
                         // store R (R_old of this edge has been read)
                         constexpr int n = PR::T.idx[e];
-                        constexpr bool partner_here = (n & 1) ? (k > 0 && PR::T.idx[e - 1] == n - 1)
-                                                              : (k + 1 < D && PR::T.idx[e + 1] == n + 1);
+                        constexpr bool partner_here = PR::partner_in_row(e, k, D);
                         if constexpr (n & 1) {
                             if constexpr (partner_here) RW[n >> 1] = __builtin_amdgcn_perm(U(rn), U(pend), 0x06020400u);
                             else RW[n >> 1] = __builtin_amdgcn_perm(U(rn), RW[n >> 1], 0x06020400u);
@@ -461,9 +467,11 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
             if (w1) ap1[col] = (float)x.y;
         }
     });
-    if (r < 8 && (u32)(nv + r) < nvp) {  // zero padding up to the next multiple of 8 columns
-        stb(STG + nv + r, 0);
-        stb(STG + nvp + nv + r, 0);
+    // zero padding up to the next multiple of 8 columns (stepping by Z: fewer than 8 lanes per
+    // codeword pair when Z < 8)
+    for (u32 p = (u32)r; (u32)nv + p < nvp; p += (u32)Z) {
+        stb(STG + nv + p, 0);
+        stb(STG + nvp + nv + p, 0);
     }
     __syncthreads();
     const int hb = a.hd_bytes;

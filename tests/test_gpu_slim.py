@@ -155,5 +155,8 @@ def test_launch_shape_reports_the_layout(nr, torch_gpu):
     code.set_launch(DEFAULT)
     assert code.launch_shape(8 * cus) == (DEFAULT, 8 * cus)
     bg2 = nr.Code(2, 208)
-    cpb = bg2.info["cw_per_block"]
-    assert bg2.launch_shape(4096) == (cpb, (4096 + cpb - 1) // cpb)
+    lay, cpb, wg = bg2.launch_layout(4096)
+    if lay == nr.LAYOUT_DUP:
+        assert wg == (4096 + cpb - 1) // cpb
+    else:
+        assert wg == min((4096 + cpb - 1) // cpb, cus)
