@@ -242,11 +242,14 @@ def bench_mixed(torch, nr, args):
     tuned = os.path.exists(TUNING_FILE)
     if tuned:
         m.load_tuning(TUNING_FILE)
-    m.plan(b["desc"])
     hd = torch.zeros(b["hd_total"], dtype=torch.uint8, device="cuda")
     its = torch.zeros(n, dtype=torch.int32, device="cuda")
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
+    # LDS layout of the tiles: timed on this batch with its flags, the faster kept
+    lay_best, lay_us = m.autotune_layout(b["desc"], llr.data_ptr(), T, hd.data_ptr(), flags=nr.EARLY_EXIT)
+    lay_name = {nr.LAYOUT_DUP: "dup", nr.LAYOUT_SLIM: "slim"}
+    m.plan(b["desc"])
     for _ in range(2):
         m.decode(llr.data_ptr(), T, hd.data_ptr(), 0, its.data_ptr(), flags=nr.EARLY_EXIT, stream=s)
     torch.cuda.synchronize()
@@ -288,6 +291,8 @@ def bench_mixed(torch, nr, args):
         "mean_iterations": round(float(its.float().mean()), 3),
         "fer": round(fe / n, 6), "ber": be / info,
         "tiling": {"source": os.path.relpath(TUNING_FILE, ROOT) if tuned else "default (most codewords that fit)",
+                   "layout": lay_name[lay_best],
+                   "layout_autotune_ms": {lay_name[k]: round(v / 1000, 3) for k, v in lay_us.items()},
                    "threads": info_["threads"], "grid": info_["grid"], "lds_bytes": info_["lds_bytes"],
                    "tiles": info_["n_tiles"], "cpb_examples": {k: cpb[k] for k in sorted(cpb)[:6]},
                    "cpb_bg1_384": cpb.get("1/384"), "cpb_bg2_208": cpb.get("2/208")},

@@ -379,7 +379,27 @@ int ldpc_mixed_set_grid(ldpc_mixed* p, int workgroups);
  * (rows beyond a codeword's parity-row count are unused).  Plan again after setting them. */
 int ldpc_mixed_set_weights(ldpc_mixed* p, int kind, int Z, int n_iter, const float* weights);
 
-/* Codewords per workgroup (LDS codeword slots) of (kind, Z); 0 = default (the most that fit).
+/* Launch layout of every tile of the object's later plans (the single-code layouts of
+ * ldpc_set_launch_layout): LDPC_LAYOUT_DUP, 512-thread workgroups with the degree-1 posteriors in
+ * registers; LDPC_LAYOUT_SLIM, 768-thread workgroups (three wavefronts per SIMD) with a single LDS
+ * read copy and the degree-1 posteriors in a device buffer of the object (one 129 KB slot per
+ * resident workgroup, allocated here); LDPC_LAYOUT_AUTO, the library's default.  Codewords per
+ * workgroup are kept per layout.  Discards the current plan and any captured graph (capture again).
+ * A set-up call: it may allocate and waits for the object's previous launch. */
+int ldpc_mixed_set_layout(ldpc_mixed* p, int layout);
+int ldpc_mixed_get_layout(const ldpc_mixed* p, int* layout);
+/* Layout autotuner for a representative batch: plans cws (n_cw descriptors addressing llr and hd,
+ * device buffers as for ldpc_mixed_decode_async), times both layouts on it with the caller's
+ * iteration count and flags (the mean of three launches after two warm-ups,
+ * microseconds per batch to us_dup / us_slim, either may be NULL) and keeps the faster
+ * (best_layout).  The early-exit decode of a realistic batch and the fixed-iteration decode can
+ * prefer different layouts: larger slim tiles run until their slowest codeword stops.  A set-up
+ * call: it synchronises the device, discards the plan and any captured graph. */
+int ldpc_mixed_autotune_layout(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n_cw, const float* llr, int n_iter,
+                               unsigned flags, uint8_t* hd, float* us_dup, float* us_slim, int* best_layout);
+
+/* Codewords per workgroup (LDS codeword slots) of (kind, Z) in the object's current layout;
+ * 0 = default (the most that fit).
  * ldpc_mixed_autotune times the candidate counts (each count that fills one more wavefront,
  * 1 and the maximum) on n_cw full-rate codewords for n_iter iterations, keeps the fastest, and
  * writes candidate k's count and microseconds per codeword to cand_cpb[k], us_per_cw[k] (k <
@@ -388,7 +408,7 @@ int ldpc_mixed_set_cpb(ldpc_mixed* p, int kind, int Z, int cpb);
 int ldpc_mixed_get_cpb(ldpc_mixed* p, int kind, int Z, int* cpb);
 int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, int* cand_cpb, float* us_per_cw,
                         int max_candidates, int* n_candidates, int* best_cpb);
-/* Tuning cache: text lines "kind Z cpb". */
+/* Tuning cache: text lines "kind Z cpb" (dup layout) and "kind Z cpb slim" (slim layout). */
 int ldpc_mixed_save_tuning(const ldpc_mixed* p, const char* path);
 int ldpc_mixed_load_tuning(ldpc_mixed* p, const char* path);
 

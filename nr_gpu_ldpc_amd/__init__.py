@@ -119,6 +119,10 @@ _sig("ldpc_mixed_replay", ctypes.c_int, _P, _P)
 _sig("ldpc_mixed_set_input", ctypes.c_int, _P, ctypes.c_int, ctypes.c_float)
 _sig("ldpc_mixed_set_grid", ctypes.c_int, _P, ctypes.c_int)
 _sig("ldpc_mixed_set_weights", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P)
+_sig("ldpc_mixed_set_layout", ctypes.c_int, _P, ctypes.c_int)
+_sig("ldpc_mixed_get_layout", ctypes.c_int, _P, _c_int_p)
+_sig("ldpc_mixed_autotune_layout", ctypes.c_int, _P, _P, ctypes.c_int, _P, ctypes.c_int, ctypes.c_uint, _P,
+     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), _c_int_p)
 _sig("ldpc_mixed_set_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int)
 _sig("ldpc_mixed_get_cpb", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _c_int_p)
 _sig("ldpc_mixed_autotune", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P,
@@ -422,6 +426,27 @@ class Mixed:
         import numpy as np
         w = np.ascontiguousarray(weights, dtype=np.float32)
         _check(lib.ldpc_mixed_set_weights(self._h, kind, Z, w.shape[0], w.ctypes.data), "ldpc_mixed_set_weights")
+
+    def set_layout(self, layout):
+        """LAYOUT_DUP, LAYOUT_SLIM or LAYOUT_AUTO for the tiles of later plans (discards the plan and
+        any captured graph)."""
+        _check(lib.ldpc_mixed_set_layout(self._h, int(layout)), "ldpc_mixed_set_layout")
+
+    def autotune_layout(self, cws, llr_ptr, n_iter, hd_ptr, flags=0):
+        """Times both layouts on a representative batch (descriptors + device LLR / HD buffers),
+        keeps the faster; returns (best layout, {layout: microseconds per batch})."""
+        import numpy as np
+        self._cws = np.ascontiguousarray(cws, dtype=MIXED_CW_DTYPE)
+        ud, us, best = ctypes.c_float(), ctypes.c_float(), ctypes.c_int()
+        _check(lib.ldpc_mixed_autotune_layout(self._h, self._cws.ctypes.data, len(self._cws), llr_ptr, n_iter, flags,
+                                              hd_ptr, ctypes.byref(ud), ctypes.byref(us), ctypes.byref(best)),
+               "ldpc_mixed_autotune_layout")
+        return best.value, {LAYOUT_DUP: ud.value, LAYOUT_SLIM: us.value}
+
+    def get_layout(self):
+        v = ctypes.c_int()
+        _check(lib.ldpc_mixed_get_layout(self._h, ctypes.byref(v)), "ldpc_mixed_get_layout")
+        return v.value
 
     def set_cpb(self, kind, Z, cpb):
         _check(lib.ldpc_mixed_set_cpb(self._h, kind, Z, cpb), "ldpc_mixed_set_cpb")

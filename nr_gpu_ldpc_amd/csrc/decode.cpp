@@ -71,6 +71,7 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     a.cpb = slim_cpb;
     a.lds_cw_bytes = (int)decoder_slim_lds_bytes_per_cw(c.kind, c.Z);
     a.d1_scratch = x->d_d1;
+    a.d1_stride = (int)(decoder_slim_slot_bytes(c.kind, c.Z, slim_cpb) / sizeof(float));
     const int grid = std::min((n_cw + slim_cpb - 1) / slim_cpb, x->cus);
     // Launches on one context share its slots: each waits for the previous one (on any stream).
     // Under stream capture the wait is not recorded: the captured graph orders its own nodes,

@@ -50,8 +50,9 @@ struct DecodeArgs {
     const int* list_count; // device count for `list`
     const uint16_t* llr_half;  // binary16 LLRs [n_cw][n_vn] for the HALF_PRC schedule (llr unused)
     const int8_t* llr_i8;      // signed-char LLRs [n_cw][n_vn] for the CHAR_PRC schedule (llr unused)
-    float* d1_scratch;         // slim-layout kernel: APP of the degree-1 columns,
-                               // [gridDim.x][cpb][ND1][Z] floats (one slot per resident workgroup)
+    float* d1_scratch;         // slim-layout kernel: APP of the degree-1 columns, one slot of
+                               // d1_stride floats per resident workgroup, [cpb][ND1][Z] used
+    int d1_stride;
 };
 
 // One codeword of a mixed batch (device table, built by the planner).
@@ -89,6 +90,8 @@ struct MixedArgs {
     int n_iter, early_exit;
     int fallback;           // learned tiles: re-decode syndrome failures with plain min-sum
     int slot_off;           // LDS byte offset of the work-queue broadcast word
+    float* d1_scratch;      // slim layout: degree-1 APP slots, d1_stride floats per workgroup
+    int d1_stride;
 };
 
 // Narrow mixed-batch LLRs (int8 q -> q * scale, or binary16) -> float, for the element count in
@@ -104,12 +107,16 @@ int llr_convert_launch(const LlrConvertArgs& a, hipStream_t s);
 int llr_convert_graph_node(const LlrConvertArgs& a, hipGraph_t g, const hipGraphNode_t* deps, size_t n_deps,
                            hipGraphNode_t* node);
 
-int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);
+// The persistent mixed kernel of one layout (slim: kLayoutSlim tiles, up to kSlimMaxBlock threads
+// at three wavefronts per SIMD, a.d1_scratch slots; else kLayoutDup tiles, up to kMaxBlock).
+int mixed_launch(bool learned, bool slim, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s);
 // The same launch as a kernel node of graph g after the given dependencies.
-int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
+int mixed_graph_node(bool learned, bool slim, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
                      const hipGraphNode_t* deps, size_t n_deps, hipGraphNode_t* node);
 // Resident workgroups per CU of the mixed kernel at this shape (0 on error).
-int mixed_occupancy(bool learned, int threads, size_t lds);
+int mixed_occupancy(bool learned, bool slim, int threads, size_t lds);
+// Floats of one workgroup's degree-1 slot in the slim mixed kernel (any tile of either base graph).
+constexpr int kMixedSlimD1Stride = 42 * kSlimMaxBlock;
 
 // LDS tilings of the fused FP32 decoder (plain and learned), the autotuner's layout axis:
 //   kLayoutDup   read copy [NND][2Z] (sub-columns stored twice: the circulant read never wraps)

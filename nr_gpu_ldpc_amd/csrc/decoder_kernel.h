@@ -265,7 +265,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && TRAIN), "mixed tiles use the plain/learned forward path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
-    static_assert(!(SLIM && (TRAIN || MIXED || HALF)), "the slim layout serves the plain and learned decodes");
+    static_assert(!(SLIM && (TRAIN || HALF)), "the slim layout serves the plain and learned decodes");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -322,7 +322,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     // slot, addressed through a buffer resource (wave-uniform base, one lane offset for all
     // slots: no per-slot 64-bit address is formed or kept)
     const __amdgpu_buffer_rsrc_t d1rs = __builtin_amdgcn_make_buffer_rsrc(
-        SLIM ? a.d1_scratch + (size_t)blockIdx.x * a.cpb * ND1 * Z : nullptr, 0, SLIM ? a.cpb * ND1 * Z * 4 : 0,
+        SLIM ? a.d1_scratch + (size_t)blockIdx.x * a.d1_stride : nullptr, 0, SLIM ? a.cpb * ND1 * Z * 4 : 0,
         kBufferDword3);
     const uint32_t d1v = SLIM ? (uint32_t)(cwl * ND1 * Z + r) * 4u : 0u;
     auto d1_ld = [&](int slot) -> float {
@@ -369,9 +369,14 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
             if constexpr (ZC > 0) c = r + edge_vm<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            if constexpr (SLIM) d1_st(S::T.d1_slot[j], llr_at(j * Z + c) + 0.0f);
-            else if (!MIXED || i < m_lane) app1[S::T.d1_slot[j]] = llr_at(j * Z + c) + 0.0f;
-            else app1[S::T.d1_slot[j]] = 0.0f;
+            // MIXED: rows beyond the codeword's own count start from 0 (their columns lie past the
+            // codeword's LLRs); lanes that shadow another lane leave its degree-1 slots alone
+            const float v = (!MIXED || i < m_lane) ? llr_at(j * Z + c) + 0.0f : 0.0f;
+            if constexpr (SLIM) {
+                if (!MIXED || !dead) d1_st(S::T.d1_slot[j], v);
+            } else {
+                app1[S::T.d1_slot[j]] = v;
+            }
         }
     });
     __syncthreads();
@@ -617,7 +622,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     if constexpr (j < NND) {
                         if (!MIXED || !dead) lds_st(addr[KC], cur[KC] + dtv[k]);
                     } else if constexpr (SLIM) {
-                        d1_st(S::T.d1_slot[j], xd1[k] + dtv[k]);
+                        if (!MIXED || !dead) d1_st(S::T.d1_slot[j], xd1[k] + dtv[k]);
                     } else {
                         float& ap = app1[S::T.d1_slot[j]];
                         ap = ap + dtv[k];

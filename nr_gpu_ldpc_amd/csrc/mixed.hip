@@ -15,23 +15,26 @@ namespace nrldpc {
 using CTile = const MixedTile __attribute__((address_space(4)));
 
 // One tile of (KIND, ZC): ZC > 0 is a lifting size with every circulant offset compiled in.
-template <int KIND, int ZC, bool LEARNED>
+// SLIM: the tile's codewords use the slim layout (single LDS read copy, degree-1 APPs in the
+// workgroup's slot of m.d1_scratch).
+template <int KIND, int ZC, bool LEARNED, bool SLIM>
 __device__ __forceinline__ void mixed_tile(const DecodeArgs& a, const MixedCw* cws, CTile* t, bool fb,
                                            uint32_t slot) {
     if (LEARNED && t->trained) {
         // learned weights; with fallback, the syndrome of the final hard decision decides which
         // codewords plain min-sum decodes again (its output replaces the learned one only where
         // it passes the check)
-        const bool ok = decode_block<KIND, ZC, true, false, true>(a, 0, cws, true, fb, false);
+        const bool ok = decode_block<KIND, ZC, true, false, true, false, SLIM>(a, 0, cws, true, fb, false);
         const bool need = fb && (int)threadIdx.x < t->count * a.Z && !ok;
-        if (fb && block_any(need, slot)) decode_block<KIND, ZC, false, false, true>(a, 0, cws, need, true, true);
+        if (fb && block_any(need, slot))
+            decode_block<KIND, ZC, false, false, true, false, SLIM>(a, 0, cws, need, true, true);
     } else {
-        decode_block<KIND, ZC, false, false, true>(a, 0, cws);
+        decode_block<KIND, ZC, false, false, true, false, SLIM>(a, 0, cws);
     }
 }
 
-template <bool LEARNED>
-__global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
+template <bool LEARNED, bool SLIM>
+__device__ __forceinline__ void mixed_body(const MixedArgs& m) {
     const uint32_t slot = lds_base() + (uint32_t)m.slot_off;
     const int n_tiles = *m.n_tiles;
     for (;;) {
@@ -59,16 +62,18 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
         a.early_exit = m.early_exit;
         a.lds_cw_bytes = t->lds_cw_bytes;
         a.nnz = kind == kBG1 ? Structure<kBG1>::NNZ : Structure<kBG2>::NNZ;
+        a.d1_scratch = m.d1_scratch;
+        a.d1_stride = m.d1_stride;
         const MixedCw* cws = m.cws + t->first;
         const bool fb = m.fallback != 0;
         // the two lifting sizes of the NR configurations that carry most traffic (BG1 Z=384,
         // BG2 Z=208) run with compiled-in offsets, every other one with the per-edge table
         if (kind == kBG1) {
-            if (a.Z == 384) mixed_tile<kBG1, 384, LEARNED>(a, cws, t, fb, slot);
-            else mixed_tile<kBG1, 0, LEARNED>(a, cws, t, fb, slot);
+            if (a.Z == 384) mixed_tile<kBG1, 384, LEARNED, SLIM>(a, cws, t, fb, slot);
+            else mixed_tile<kBG1, 0, LEARNED, SLIM>(a, cws, t, fb, slot);
         } else {
-            if (a.Z == 208) mixed_tile<kBG2, 208, LEARNED>(a, cws, t, fb, slot);
-            else mixed_tile<kBG2, 0, LEARNED>(a, cws, t, fb, slot);
+            if (a.Z == 208) mixed_tile<kBG2, 208, LEARNED, SLIM>(a, cws, t, fb, slot);
+            else mixed_tile<kBG2, 0, LEARNED, SLIM>(a, cws, t, fb, slot);
         }
     }
     if (threadIdx.x == 0) {
@@ -80,27 +85,52 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
     }
 }
 
-int mixed_launch(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s) {
-    if (threads > kMaxBlock || threads < 64 || lds > (size_t)max_lds_bytes() || grid < 1) {
+template <bool LEARNED>
+__global__ void __launch_bounds__(kMaxBlock) k_decode_mixed(MixedArgs m) {
+    mixed_body<LEARNED, false>(m);
+}
+
+// The slim layout: up to 12 wavefronts per workgroup at three per SIMD (one workgroup per CU).
+template <bool LEARNED>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kSlimMaxBlock), amdgpu_waves_per_eu(3)))
+k_decode_mixed_slim(MixedArgs m) {
+    mixed_body<LEARNED, true>(m);
+}
+
+template <bool LEARNED>
+static void* mixed_kernel(bool slim) {
+    return slim ? reinterpret_cast<void*>(k_decode_mixed_slim<LEARNED>) : reinterpret_cast<void*>(k_decode_mixed<LEARNED>);
+}
+
+static bool mixed_shape_ok(bool slim, const MixedArgs& a, int grid, int threads, size_t lds) {
+    if (threads > (slim ? kSlimMaxBlock : kMaxBlock) || threads < 64 || lds > (size_t)max_lds_bytes() || grid < 1 ||
+        (slim && (!a.d1_scratch || a.d1_stride < kMixedSlimD1Stride))) {
         set_last_error("mixed launch exceeds the workgroup size or LDS limit");
-        return LDPC_EINVAL;
+        return false;
     }
-    if (learned) hipLaunchKernelGGL(k_decode_mixed<true>, dim3(grid), dim3(threads), lds, s, a);
-    else hipLaunchKernelGGL(k_decode_mixed<false>, dim3(grid), dim3(threads), lds, s, a);
+    return true;
+}
+
+int mixed_launch(bool learned, bool slim, const MixedArgs& a, int grid, int threads, size_t lds, hipStream_t s) {
+    if (!mixed_shape_ok(slim, a, grid, threads, lds)) return LDPC_EINVAL;
+    if (slim) {
+        if (learned) hipLaunchKernelGGL(k_decode_mixed_slim<true>, dim3(grid), dim3(threads), lds, s, a);
+        else hipLaunchKernelGGL(k_decode_mixed_slim<false>, dim3(grid), dim3(threads), lds, s, a);
+    } else {
+        if (learned) hipLaunchKernelGGL(k_decode_mixed<true>, dim3(grid), dim3(threads), lds, s, a);
+        else hipLaunchKernelGGL(k_decode_mixed<false>, dim3(grid), dim3(threads), lds, s, a);
+    }
     NRLDPC_HIP(hipGetLastError());
     return LDPC_OK;
 }
 
-int mixed_graph_node(bool learned, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
+int mixed_graph_node(bool learned, bool slim, const MixedArgs& a, int grid, int threads, size_t lds, hipGraph_t g,
                      const hipGraphNode_t* deps, size_t n_deps, hipGraphNode_t* node) {
-    if (threads > kMaxBlock || threads < 64 || lds > (size_t)max_lds_bytes() || grid < 1) {
-        set_last_error("mixed launch exceeds the workgroup size or LDS limit");
-        return LDPC_EINVAL;
-    }
+    if (!mixed_shape_ok(slim, a, grid, threads, lds)) return LDPC_EINVAL;
     MixedArgs args = a;  // the node keeps its own copy of the argument values
     void* params[] = {&args};
     hipKernelNodeParams kp{};
-    kp.func = learned ? reinterpret_cast<void*>(k_decode_mixed<true>) : reinterpret_cast<void*>(k_decode_mixed<false>);
+    kp.func = learned ? mixed_kernel<true>(slim) : mixed_kernel<false>(slim);
     kp.gridDim = dim3(grid);
     kp.blockDim = dim3(threads);
     kp.sharedMemBytes = (unsigned)lds;
@@ -143,10 +173,13 @@ int llr_convert_graph_node(const LlrConvertArgs& a, hipGraph_t g, const hipGraph
     return LDPC_OK;
 }
 
-int mixed_occupancy(bool learned, int threads, size_t lds) {
+int mixed_occupancy(bool learned, bool slim, int threads, size_t lds) {
     int n = 0;
-    hipError_t e = learned ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<true>, threads, lds)
-                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<false>, threads, lds);
+    hipError_t e;
+    if (slim) e = learned ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed_slim<true>, threads, lds)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed_slim<false>, threads, lds);
+    else e = learned ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<true>, threads, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_decode_mixed<false>, threads, lds);
     return e == hipSuccess ? n : 0;
 }
 

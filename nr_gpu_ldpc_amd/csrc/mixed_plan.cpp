@@ -28,14 +28,18 @@
 
 namespace nrldpc {
 
-constexpr int kMixedThreads = 512;  // workgroup size of every mixed launch
+constexpr int kMixedThreads = 512;  // workgroup size of the dup-layout mixed kernel
+// layout of a new object (and of LDPC_LAYOUT_AUTO)
+constexpr int kMixedDefaultLayout = kLayoutDup;
 constexpr size_t kHdrBytes = 256;
 
 struct MixedCode {
     CodeDesc code;                 // whole base graph
     EdgeParam* d_etab = nullptr;
-    int lds_cw = 0;                // LDS bytes per codeword slot
-    int cpb_max = 1, cpb_default = 1, cpb = 0;  // cpb 0: default
+    // per layout of the object (index 0: dup, 1: slim): LDS bytes per codeword slot and codewords
+    // per workgroup (cpb 0: the default, the most that fit)
+    int lds_cw[2] = {0, 0};
+    int cpb_max[2] = {1, 1}, cpb_default[2] = {1, 1}, cpb[2] = {0, 0};
     float2* d_weights = nullptr;   // trained weights [(T+1)][nnz_full], row 0 zero
     int weights_iters = 0;
 };
@@ -56,6 +60,12 @@ using namespace nrldpc;
 struct ldpc_mixed {
     int device = -1;
     int max_cw = 0;
+    // launch layout of every tile (nrldpc::kLayoutDup: 512-thread workgroups, degree-1 APPs in
+    // registers; kLayoutSlim: 768-thread workgroups at three wavefronts per SIMD, single LDS read
+    // copy, degree-1 APPs in d_d1)
+    int layout = kLayoutDup;
+    float* d_d1 = nullptr;             // slim: kMixedSlimD1Stride floats per resident workgroup
+    int d1_slots = 0;
     int n_cu = 256, grid = 256, threads = kMixedThreads;
     int grid_full = 256;               // resident workgroups of the whole chip (the default grid)
     int llr_fmt = kLlrF32;             // element type of the llr buffers of later launches
@@ -107,14 +117,17 @@ static int mixed_get_code(ldpc_mixed* p, int kind, int Z, MixedCode** out) {
     std::string err;
     int st = make_code(kind, Z, 0, &mc.code, &err);
     if (st) { set_last_error(err); return st; }
-    mc.lds_cw = (int)decoder_lds_bytes_per_cw(kind, Z);
     const int slots = p->lds - 32;  // workgroup flag and queue word above the codeword slots
-    mc.cpb_max = std::max(1, std::min(p->threads / Z, slots / mc.lds_cw));
-    if (mc.lds_cw > slots || Z > p->threads) {
+    for (int L = 0; L < 2; L++) {
+        const int threads = L ? kSlimMaxBlock : kMixedThreads;
+        mc.lds_cw[L] = (int)(L ? decoder_slim_lds_bytes_per_cw(kind, Z) : decoder_lds_bytes_per_cw(kind, Z));
+        mc.cpb_max[L] = std::max(1, std::min(threads / Z, slots / mc.lds_cw[L]));
+        mc.cpb_default[L] = mc.cpb_max[L];  // the most lanes in use (the autotuner may choose fewer)
+    }
+    if (mc.lds_cw[0] > slots || Z > kMixedThreads) {
         set_last_error("lifting size does not fit one workgroup");
         return LDPC_EINVAL;
     }
-    mc.cpb_default = mc.cpb_max;  // the most lanes in use (the autotuner may choose fewer)
     if (p->device >= 0) {
         std::vector<EdgeParam> et;
         build_edge_params(mc.code, &et);
@@ -128,8 +141,11 @@ static int mixed_get_code(ldpc_mixed* p, int kind, int Z, MixedCode** out) {
 
 static size_t llr_elem_bytes(int fmt) { return fmt == kLlrI8 ? 1 : fmt == kLlrF16 ? 2 : 4; }
 
-static int mixed_cpb(const MixedCode& mc) {
-    return mc.cpb > 0 ? std::min(mc.cpb, mc.cpb_max) : mc.cpb_default;
+static int lay_index(const ldpc_mixed* p) { return p->layout == kLayoutSlim ? 1 : 0; }
+
+static int mixed_cpb(const ldpc_mixed* p, const MixedCode& mc) {
+    const int L = lay_index(p);
+    return mc.cpb[L] > 0 ? std::min(mc.cpb[L], mc.cpb_max[L]) : mc.cpb_default[L];
 }
 
 static int wait_upload(ldpc_mixed* p) {
@@ -185,7 +201,7 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
         MixedCode& mc = p->codes[g.first];
         std::vector<int>& v = g.second;
         std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return cws[a].m_rows > cws[b].m_rows; });
-        const int cpb = mixed_cpb(mc);
+        const int cpb = mixed_cpb(p, mc);
         if (mc.d_weights) {
             if (trained_T == 0) trained_T = mc.weights_iters;
             else if (trained_T != mc.weights_iters) trained_T = -1;
@@ -201,7 +217,7 @@ static int mixed_plan_locked(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n) {
             h.t.count = (int)std::min<size_t>(cpb, v.size() - s);
             h.t.cpb = h.t.count;  // a short last tile uses only the lanes it needs
             h.t.first = (int)(slot_of.size());
-            h.t.lds_cw_bytes = mc.lds_cw;
+            h.t.lds_cw_bytes = mc.lds_cw[lay_index(p)];
             for (int k = 0; k < h.t.count; k++) slot_of.push_back(v[s + k]);
             const int waves = (h.t.cpb * mc.code.Z + 63) / 64;
             h.cost = (long long)((waves + 3) / 4) * edges_in_rows(h.t.kind, h.t.m_rows);
@@ -301,6 +317,8 @@ static int mixed_args(ldpc_mixed* p, const float* llr, int n_iter, unsigned flag
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.fallback = (learned && (flags & LDPC_FLAG_FALLBACK_PLAIN)) ? 1 : 0;
     a.slot_off = p->lds - 16;
+    a.d1_scratch = p->d_d1;
+    a.d1_stride = kMixedSlimD1Stride;
     return LDPC_OK;
 }
 
@@ -344,7 +362,8 @@ static int mixed_enqueue(ldpc_mixed* p, hipStream_t s, const float* llr, int n_i
     NRLDPC_HIP(hipMemcpyAsync(p->d_plan + p->cws_off, p->h_plan + p->cws_off, plan_body_bytes(p),
                               hipMemcpyHostToDevice, s));
     if (narrow && (st = llr_convert_launch(convert_args(p, llr), s))) return st;
-    return mixed_launch((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, s);
+    return mixed_launch((flags & LDPC_FLAG_LEARNED) != 0, p->layout == kLayoutSlim, a, p->grid, p->threads,
+                        (size_t)p->lds, s);
 }
 
 }  // namespace nrldpc
@@ -378,7 +397,7 @@ int ldpc_mixed_create(ldpc_mixed** out, int device, int max_cw) {
     p->lds = max_lds_bytes() & ~15;
     if (hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
         return fail(LDPC_EHIP);
-    const int occ = std::max(1, mixed_occupancy(false, p->threads, (size_t)p->lds));
+    const int occ = std::max(1, mixed_occupancy(false, false, p->threads, (size_t)p->lds));
     p->grid = p->grid_full = p->n_cu * occ;
     if (hipHostMalloc(reinterpret_cast<void**>(&p->h_plan), p->plan_bytes, hipHostMallocDefault) != hipSuccess ||
         hipMalloc(reinterpret_cast<void**>(&p->d_plan), p->plan_bytes) != hipSuccess ||
@@ -410,6 +429,7 @@ int ldpc_mixed_destroy(ldpc_mixed* p) {
         if (p->d_plan) (void)hipFree(p->d_plan);
         if (p->d_conv) (void)hipFree(p->d_conv);
         if (p->d_queue) (void)hipFree(p->d_queue);
+        if (p->d_d1) (void)hipFree(p->d_d1);
         if (p->h_plan) (void)hipHostFree(p->h_plan);
     } else {
         free(p->h_plan);
@@ -505,8 +525,8 @@ int ldpc_mixed_capture(ldpc_mixed* p, void* stream, const float* llr, int n_iter
         if ((st = llr_convert_graph_node(convert_args(p, llr), g, &p->n_body, 1, &conv))) return st;
         before = conv;
     }
-    if ((st = mixed_graph_node((flags & LDPC_FLAG_LEARNED) != 0, a, p->grid, p->threads, (size_t)p->lds, g,
-                               &before, 1, &kern)))
+    if ((st = mixed_graph_node((flags & LDPC_FLAG_LEARNED) != 0, p->layout == kLayoutSlim, a, p->grid, p->threads,
+                               (size_t)p->lds, g, &before, 1, &kern)))
         return st;
     (void)kern;
     p->g_llr_host = llr_host;
@@ -597,19 +617,96 @@ int ldpc_mixed_set_grid(ldpc_mixed* p, int workgroups) {
     return LDPC_OK;
 }
 
+int ldpc_mixed_set_layout(ldpc_mixed* p, int layout) {
+    if (!p || (layout != LDPC_LAYOUT_AUTO && layout != LDPC_LAYOUT_DUP && layout != LDPC_LAYOUT_SLIM)) {
+        set_last_error("mixed layout: LDPC_LAYOUT_AUTO, LDPC_LAYOUT_DUP or LDPC_LAYOUT_SLIM");
+        return LDPC_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    const int lay = layout == LDPC_LAYOUT_AUTO ? kMixedDefaultLayout : layout;
+    const bool slim = lay == kLayoutSlim;
+    const int threads = slim ? kSlimMaxBlock : kMixedThreads;
+    if (p->device >= 0) {
+        NRLDPC_HIP(hipSetDevice(p->device));
+        int st = wait_upload(p);
+        if (st) return st;
+        const int occ = std::max(1, mixed_occupancy(false, slim, threads, (size_t)p->lds));
+        const int grid_full = p->n_cu * occ;
+        if (slim && p->d1_slots < grid_full) {
+            if (p->d_d1) NRLDPC_HIP(hipFree(p->d_d1));
+            p->d_d1 = nullptr;
+            p->d1_slots = 0;
+            NRLDPC_HIP(hipMalloc(reinterpret_cast<void**>(&p->d_d1), (size_t)grid_full * kMixedSlimD1Stride * sizeof(float)));
+            p->d1_slots = grid_full;
+        }
+        // a captured graph holds the other layout's kernel: capture again
+        if (p->exec) { NRLDPC_HIP(hipGraphExecDestroy(p->exec)); p->exec = nullptr; }
+        if (p->graph) { NRLDPC_HIP(hipGraphDestroy(p->graph)); p->graph = nullptr; }
+        p->grid = p->grid_full = grid_full;
+    }
+    p->threads = threads;
+    p->layout = lay;
+    p->planned = false;  // tiles hold the layout's codewords per workgroup
+    return LDPC_OK;
+}
+
+int ldpc_mixed_autotune_layout(ldpc_mixed* p, const ldpc_mixed_cw* cws, int n_cw, const float* llr, int n_iter,
+                               unsigned flags, uint8_t* hd, float* us_dup, float* us_slim, int* best_layout) {
+    if (!p || !cws || n_cw < 1 || !llr || !hd || n_iter < 1) return LDPC_EINVAL;
+    if (p->device < 0) { set_last_error("host-only mixed object"); return LDPC_ENODEV; }
+    float us[2] = {0.f, 0.f};
+    const int lays[2] = {LDPC_LAYOUT_DUP, LDPC_LAYOUT_SLIM};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = LDPC_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) st = LDPC_EHIP;
+    for (int L = 0; L < 2 && st == LDPC_OK; L++) {
+        if ((st = ldpc_mixed_set_layout(p, lays[L]))) break;
+        std::lock_guard<std::mutex> lk(p->mu);
+        if ((st = mixed_plan_locked(p, cws, n_cw))) break;
+        for (int rep = 0; rep < 5 && st == LDPC_OK; rep++) {  // two warm-up launches, three timed
+            if (rep == 2 && hipEventRecord(e0, nullptr) != hipSuccess) st = LDPC_EHIP;
+            if (st == LDPC_OK) st = mixed_enqueue(p, nullptr, llr, n_iter, flags, hd, nullptr, nullptr);
+        }
+        float ms = 0.f;
+        if (st == LDPC_OK && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            st = LDPC_EHIP;
+        us[L] = ms * 1000.f / 3.f;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    const int best = us[1] < us[0] ? LDPC_LAYOUT_SLIM : LDPC_LAYOUT_DUP;
+    const int st2 = ldpc_mixed_set_layout(p, st == LDPC_OK ? best : LDPC_LAYOUT_AUTO);
+    if (st) return st;
+    if (us_dup) *us_dup = us[0];
+    if (us_slim) *us_slim = us[1];
+    if (best_layout) *best_layout = best;
+    return st2;
+}
+
+int ldpc_mixed_get_layout(const ldpc_mixed* p, int* layout) {
+    if (!p || !layout) return LDPC_EINVAL;
+    *layout = p->layout;
+    return LDPC_OK;
+}
+
+static int set_cpb_locked(ldpc_mixed* p, int kind, int Z, int cpb, int L) {
+    MixedCode* mc = nullptr;
+    int st = mixed_get_code(p, kind, Z, &mc);
+    if (st) return st;
+    if (cpb > mc->cpb_max[L]) {
+        set_last_error("codewords per workgroup exceed the workgroup size or LDS");
+        return LDPC_EINVAL;
+    }
+    mc->cpb[L] = cpb;
+    return LDPC_OK;
+}
+
 int ldpc_mixed_set_cpb(ldpc_mixed* p, int kind, int Z, int cpb) {
     if (!p || cpb < 0) return LDPC_EINVAL;
     std::lock_guard<std::mutex> lk(p->mu);
     if (p->device >= 0) NRLDPC_HIP(hipSetDevice(p->device));
-    MixedCode* mc = nullptr;
-    int st = mixed_get_code(p, kind, Z, &mc);
-    if (st) return st;
-    if (cpb > mc->cpb_max) {
-        set_last_error("codewords per workgroup exceed the workgroup size or LDS");
-        return LDPC_EINVAL;
-    }
-    mc->cpb = cpb;
-    return LDPC_OK;
+    return set_cpb_locked(p, kind, Z, cpb, lay_index(p));
 }
 
 int ldpc_mixed_get_cpb(ldpc_mixed* p, int kind, int Z, int* cpb) {
@@ -619,7 +716,7 @@ int ldpc_mixed_get_cpb(ldpc_mixed* p, int kind, int Z, int* cpb) {
     MixedCode* mc = nullptr;
     int st = mixed_get_code(p, kind, Z, &mc);
     if (st) return st;
-    *cpb = mixed_cpb(*mc);
+    *cpb = mixed_cpb(p, *mc);
     return LDPC_OK;
 }
 
@@ -632,6 +729,7 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
     MixedCode* mc = nullptr;
     int st = mixed_get_code(p, kind, Z, &mc);
     if (st) return st;
+    const int L = lay_index(p);
     const int nvn = mc->code.n_vn(), hb = mc->code.hd_bytes();
     std::vector<ldpc_mixed_cw> d(n_cw);
     for (int i = 0; i < n_cw; i++)
@@ -639,8 +737,8 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
     // Candidates: every count that fills one more wavefront, plus 1 and the maximum.  Without
     // early exit the work is data independent, so all-zero LLRs time the real decoder.
     std::vector<int> cand{1};
-    for (int k = 1; k <= p->threads / 64; k++) cand.push_back(std::max(1, std::min(mc->cpb_max, 64 * k / Z)));
-    cand.push_back(mc->cpb_max);
+    for (int k = 1; k <= p->threads / 64; k++) cand.push_back(std::max(1, std::min(mc->cpb_max[L], 64 * k / Z)));
+    cand.push_back(mc->cpb_max[L]);
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     float* llr = nullptr;
@@ -660,13 +758,13 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
         set_last_error("autotune: allocation failed");
         return LDPC_ENOMEM;
     }
-    const int saved = mc->cpb;
+    const int saved = mc->cpb[L];
     const int saved_fmt = p->llr_fmt;  // the candidates are timed on float LLRs
     p->llr_fmt = kLlrF32;
     int best = 1;
     float best_us = 3.4e38f;
     for (size_t c = 0; c < cand.size() && st == LDPC_OK; c++) {
-        mc->cpb = cand[c];
+        mc->cpb[L] = cand[c];
         if ((st = mixed_plan_locked(p, d.data(), n_cw))) break;
         float ms = 0.f;
         if ((st = order_after_previous(p, nullptr))) break;
@@ -691,8 +789,8 @@ int ldpc_mixed_autotune(ldpc_mixed* p, int kind, int Z, int n_iter, int n_cw, in
     cleanup();
     p->llr_fmt = saved_fmt;
     p->planned = false;
-    if (st) { mc->cpb = saved; return st; }
-    mc->cpb = best;
+    if (st) { mc->cpb[L] = saved; return st; }
+    mc->cpb[L] = best;
     if (n_candidates) *n_candidates = (int)cand.size();
     if (best_cpb) *best_cpb = best;
     return LDPC_OK;
@@ -702,9 +800,11 @@ int ldpc_mixed_save_tuning(const ldpc_mixed* p, const char* path) {
     if (!p || !path) return LDPC_EINVAL;
     FILE* f = fopen(path, "w");
     if (!f) { set_last_error(std::string("cannot write ") + path); return LDPC_EINVAL; }
-    fprintf(f, "# kind Z cpb (codewords per workgroup of the mixed-batch decoder)\n");
-    for (const auto& c : p->codes)
-        if (c.second.cpb > 0) fprintf(f, "%d %d %d\n", c.second.code.kind, c.second.code.Z, c.second.cpb);
+    fprintf(f, "# kind Z cpb [slim] (codewords per workgroup of the mixed-batch decoder; \"slim\": the slim layout)\n");
+    for (int L = 0; L < 2; L++)
+        for (const auto& c : p->codes)
+            if (c.second.cpb[L] > 0)
+                fprintf(f, "%d %d %d%s\n", c.second.code.kind, c.second.code.Z, c.second.cpb[L], L ? " slim" : "");
     fclose(f);
     return LDPC_OK;
 }
@@ -713,17 +813,21 @@ int ldpc_mixed_load_tuning(ldpc_mixed* p, const char* path) {
     if (!p || !path) return LDPC_EINVAL;
     FILE* f = fopen(path, "r");
     if (!f) { set_last_error(std::string("cannot read ") + path); return LDPC_EINVAL; }
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (p->device >= 0 && hipSetDevice(p->device) != hipSuccess) { fclose(f); return LDPC_EHIP; }
     char line[256];
     int st = LDPC_OK;
     while (st == LDPC_OK && fgets(line, sizeof line, f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         int kind = 0, Z = 0, cpb = 0;
-        if (sscanf(line, "%d %d %d", &kind, &Z, &cpb) != 3) {
+        char lay[16] = {0};
+        const int got = sscanf(line, "%d %d %d %15s", &kind, &Z, &cpb, lay);
+        if (got < 3 || cpb < 0 || (got == 4 && strcmp(lay, "slim") != 0)) {
             set_last_error(std::string("malformed tuning line: ") + line);
             st = LDPC_EINVAL;
             break;
         }
-        st = ldpc_mixed_set_cpb(p, kind, Z, cpb);
+        st = set_cpb_locked(p, kind, Z, cpb, got == 4 ? 1 : 0);
     }
     fclose(f);
     return st;

@@ -38,19 +38,26 @@ def batch_for(Z, cpb_max, n_cu=256, cap=65536):
     return int(min(cap, max(1024, 4 * n_cu * cpb_max)))
 
 
-def tune_all(iters=20, kinds=(nr.BG1, nr.BG2), sizes=None, device=0, log=print):
+def tune_all(iters=20, kinds=(nr.BG1, nr.BG2), sizes=None, device=0, log=print,
+             layouts=(nr.LAYOUT_DUP, nr.LAYOUT_SLIM)):
+    """Both layouts' codewords per workgroup for every (base graph, Z); the object ends in the
+    first layout, holding the tuned counts of all."""
     sizes = LIFTING_SIZES if sizes is None else sizes
     m = nr.Mixed(65536, device)
     table = {}
     t0 = time.time()
-    for kind in kinds:
-        for Z in sizes:
-            m.set_cpb(kind, Z, 0)
-            cmax = m.get_cpb(kind, Z)
-            n = batch_for(Z, cmax)
-            best, times = m.autotune(kind, Z, iters, n)
-            table[f"{kind}/{Z}"] = {"best": best, "n_cw": n, "us_per_cw": {str(k): round(v, 4) for k, v in times.items()}}
-            log(f"kind {kind} Z {Z}: best cpb {best} of {sorted(times)} ({time.time() - t0:.1f} s)")
+    for layout in reversed(layouts):
+        m.set_layout(layout)
+        name = "slim" if layout == nr.LAYOUT_SLIM else "dup"
+        for kind in kinds:
+            for Z in sizes:
+                m.set_cpb(kind, Z, 0)
+                cmax = m.get_cpb(kind, Z)
+                n = batch_for(Z, cmax)
+                best, times = m.autotune(kind, Z, iters, n)
+                table[f"{name}/{kind}/{Z}"] = {"best": best, "n_cw": n,
+                                               "us_per_cw": {str(k): round(v, 4) for k, v in times.items()}}
+                log(f"{name} kind {kind} Z {Z}: best cpb {best} of {sorted(times)} ({time.time() - t0:.1f} s)")
     return m, table
 
 

@@ -98,3 +98,29 @@ def test_tuning_file_roundtrip(nr, tmp_path):
     path.write_text("1 64\n")
     with pytest.raises(nr.LdpcError):
         q.load_tuning(str(path))
+
+
+def test_tuning_file_keeps_both_layouts(nr, tmp_path):
+    """Codewords per workgroup are kept per layout; the tuning file holds both ("slim" lines)."""
+    p = _planner(nr)
+    p.set_cpb(1, 64, 5)
+    p.set_layout(nr.LAYOUT_SLIM)
+    assert p.plan_info()["threads"] == 768
+    assert p.get_cpb(1, 64) == 768 // 64     # the slim layout's own default
+    p.set_cpb(1, 64, 9)
+    p.set_cpb(1, 384, 2)                     # two BG1 Z=384 codewords fit 768 lanes
+    path = tmp_path / "tune.txt"
+    p.save_tuning(str(path))
+    assert "1 64 9 slim" in path.read_text() and "1 64 5\n" in path.read_text()
+    q = _planner(nr)
+    q.load_tuning(str(path))
+    assert q.get_cpb(1, 64) == 5
+    q.set_layout(nr.LAYOUT_SLIM)
+    assert q.get_cpb(1, 64) == 9 and q.get_cpb(1, 384) == 2
+    q.set_layout(nr.LAYOUT_AUTO)
+    assert q.get_layout() == nr.LAYOUT_DUP and q.get_cpb(1, 64) == 5
+    path.write_text("1 64 3 fast\n")
+    with pytest.raises(nr.LdpcError):
+        q.load_tuning(str(path))
+    with pytest.raises(nr.LdpcError):
+        q.set_layout(7)
