@@ -87,7 +87,8 @@ int ldpc_decode_async(ldpc_ctx* ctx, void* stream, const float* llr, int n_cw, i
  * R, dt and the posterior between iterations stored in binary16 where the reference stores them
  * (src/ldpc_kernels.cu HALF_PRC branches: :158-160, :258-266, :304-306, :351, :385-386), all
  * arithmetic in single precision; outputs as for ldpc_decode_async (app_out: the final
- * single-precision posterior). */
+ * single-precision posterior).  It runs in the launch layout a single-precision decode of the
+ * same batch would use (ldpc_set_launch_layout). */
 int ldpc_decode_half_async(ldpc_ctx* ctx, void* stream, const uint16_t* llr_half, int n_cw, int n_iter,
                            unsigned flags, uint8_t* hd_packed, float* app_out, int* iters_out);
 /* The reference's CHAR_PRC build (its default, src/globals.h:12-13): signed-char channel LLRs

@@ -67,6 +67,8 @@ int autotune_layouts(ldpc_ctx* x, int n_iter, int max_candidates, int* cand_layo
 int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,
                 uint8_t* hd, float* app_out, int* iters_out, const float2* weights, const int* list,
                 const int* list_count, int* fail_out = nullptr);
+// The slim-layout launch of a (a.n_cw codewords) on x's slots, ordered after x's previous one.
+int launch_slim(ldpc_ctx* x, DecodeArgs& a, bool learned, bool half, hipStream_t s);
 int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,
                 float* app_out, int* iters_out);
 int decode_char(ldpc_ctx* x, void* stream, const int8_t* llr, int n_cw, int n_iter, unsigned flags, uint8_t* hd,

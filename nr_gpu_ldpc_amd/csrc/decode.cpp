@@ -67,12 +67,17 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     // The slim layout: explicitly chosen (ldpc_set_launch with its codewords per workgroup), or
     // by batch size when the launch shape is left to the context.
     if (!slim_chosen(x, n_cw)) return decoder_launch(c.kind, weights != nullptr, a, s);
+    return launch_slim(x, a, weights != nullptr, false, s);
+}
+
+int launch_slim(ldpc_ctx* x, DecodeArgs& a, bool learned, bool half, hipStream_t s) {
+    const CodeDesc& c = x->code;
     const int slim_cpb = x->slim_cpb;
     a.cpb = slim_cpb;
     a.lds_cw_bytes = (int)decoder_slim_lds_bytes_per_cw(c.kind, c.Z);
     a.d1_scratch = x->d_d1;
     a.d1_stride = (int)(decoder_slim_slot_bytes(c.kind, c.Z, slim_cpb) / sizeof(float));
-    const int grid = std::min((n_cw + slim_cpb - 1) / slim_cpb, x->cus);
+    const int grid = std::min((a.n_cw + slim_cpb - 1) / slim_cpb, x->cus);
     // Launches on one context share its slots: each waits for the previous one (on any stream).
     // Under stream capture the wait is not recorded: the captured graph orders its own nodes,
     // and graphs that share one context must not run concurrently.
@@ -81,7 +86,7 @@ int decode_with(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_ite
     NRLDPC_HIP(hipStreamIsCapturing(s, &cs));
     const bool capturing = cs != hipStreamCaptureStatusNone;
     if (!capturing) NRLDPC_HIP(hipStreamWaitEvent(s, x->d1_done, 0));
-    const int st = decoder_launch_slim(c.kind, weights != nullptr, a, grid, s);
+    const int st = decoder_launch_slim(c.kind, learned, a, grid, s, half);
     if (!capturing) NRLDPC_HIP(hipEventRecord(x->d1_done, s));
     return st;
 }
@@ -107,6 +112,8 @@ int decode_half(ldpc_ctx* x, void* stream, const uint16_t* llr, int n_cw, int n_
     a.early_exit = (flags & LDPC_FLAG_EARLY_EXIT) ? 1 : 0;
     a.lds_cw_bytes = (int)decoder_lds_bytes_per_cw(c.kind, c.Z);
     a.nnz = nnz_full(c.kind);
+    // the same layout choice as the single-precision decode of this batch
+    if (slim_chosen(x, n_cw)) return launch_slim(x, a, false, true, static_cast<hipStream_t>(stream));
     return decoder_launch_half(c.kind, a, static_cast<hipStream_t>(stream));
 }
 

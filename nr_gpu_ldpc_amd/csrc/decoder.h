@@ -135,8 +135,9 @@ size_t decoder_slim_lds_bytes_per_cw(int kind, int Z);
 int decoder_dup_cpb_max(int kind, int Z);
 int decoder_slim_cpb_max(int kind, int Z);
 size_t decoder_slim_slot_bytes(int kind, int Z, int cpb);  // a.d1_scratch bytes per workgroup slot
-// Persistent slim launch of `grid` workgroups of a.cpb codewords (a.d1_scratch holds grid slots).
-int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s);
+// Persistent slim launch of `grid` workgroups of a.cpb codewords (a.d1_scratch holds grid slots);
+// half: the HALF_PRC schedule (binary16 LLRs in a.llr_half, plain min-sum).
+int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s, bool half = false);
 // Resident workgroups per CU of the dup-layout kernel at this shape (0 on error).
 int decoder_dup_occupancy(int kind, int Z, bool learned, int threads, size_t lds);
 

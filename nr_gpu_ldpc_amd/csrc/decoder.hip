@@ -18,7 +18,7 @@ int launch_decode_train(const DecodeArgs& a, int blocks, int threads, size_t lds
 
 template <int KIND, int ZC>
 int launch_decode_half(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
-template <int KIND, int ZC, bool LEARNED>
+template <int KIND, int ZC, bool LEARNED, bool HALF>
 int launch_decode_slim(const DecodeArgs& a, int grid, int threads, size_t lds, hipStream_t s);
 template <int KIND, int ZC>
 int launch_decode_char_pair(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
@@ -104,30 +104,32 @@ int decoder_dup_occupancy(int kind, int Z, bool learned, int threads, size_t lds
     return learned ? occupancy_decode<kWifi, 0, true>(threads, lds) : occupancy_decode<kWifi, 0, false>(threads, lds);
 }
 
-int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s) {
+// One code kind's slim kernels: compiled-in offsets at ZS, the per-edge table elsewhere.
+template <int KIND, int ZS>
+static int slim_for_kind(bool learned, bool half, const DecodeArgs& a, int grid, int threads, size_t lds,
+                         hipStream_t s) {
+    if (a.Z == ZS) {
+        if (half) return launch_decode_slim<KIND, ZS, false, true>(a, grid, threads, lds, s);
+        return learned ? launch_decode_slim<KIND, ZS, true, false>(a, grid, threads, lds, s)
+                       : launch_decode_slim<KIND, ZS, false, false>(a, grid, threads, lds, s);
+    }
+    if (half) return launch_decode_slim<KIND, 0, false, true>(a, grid, threads, lds, s);
+    return learned ? launch_decode_slim<KIND, 0, true, false>(a, grid, threads, lds, s)
+                   : launch_decode_slim<KIND, 0, false, false>(a, grid, threads, lds, s);
+}
+
+int decoder_launch_slim(int kind, bool learned, const DecodeArgs& a, int grid, hipStream_t s, bool half) {
     const int threads = a.cpb * a.Z;
     const size_t lds = (size_t)a.cpb * a.lds_cw_bytes + 16;  // + workgroup flag slot
     if (a.cpb < 1 || a.cpb > decoder_slim_cpb_max(kind, a.Z) || threads > kSlimMaxBlock ||
-        lds > (size_t)max_lds_bytes() || grid < 1 || (n_nd1(kind) > 0 && !a.d1_scratch)) {
+        lds > (size_t)max_lds_bytes() || grid < 1 || (n_nd1(kind) > 0 && !a.d1_scratch) || (half && learned) ||
+        (half && !a.llr_half)) {
         set_last_error("slim decoder launch: unsupported shape");
         return LDPC_EINVAL;
     }
-    if (kind == kBG1) {
-        if (a.Z == kBG1Special) return learned ? launch_decode_slim<kBG1, kBG1Special, true>(a, grid, threads, lds, s)
-                                               : launch_decode_slim<kBG1, kBG1Special, false>(a, grid, threads, lds, s);
-        return learned ? launch_decode_slim<kBG1, 0, true>(a, grid, threads, lds, s)
-                       : launch_decode_slim<kBG1, 0, false>(a, grid, threads, lds, s);
-    }
-    if (kind == kBG2) {
-        if (a.Z == kBG2Special) return learned ? launch_decode_slim<kBG2, kBG2Special, true>(a, grid, threads, lds, s)
-                                               : launch_decode_slim<kBG2, kBG2Special, false>(a, grid, threads, lds, s);
-        return learned ? launch_decode_slim<kBG2, 0, true>(a, grid, threads, lds, s)
-                       : launch_decode_slim<kBG2, 0, false>(a, grid, threads, lds, s);
-    }
-    if (a.Z == kQCSpecial) return learned ? launch_decode_slim<kWifi, kQCSpecial, true>(a, grid, threads, lds, s)
-                                          : launch_decode_slim<kWifi, kQCSpecial, false>(a, grid, threads, lds, s);
-    return learned ? launch_decode_slim<kWifi, 0, true>(a, grid, threads, lds, s)
-                   : launch_decode_slim<kWifi, 0, false>(a, grid, threads, lds, s);
+    if (kind == kBG1) return slim_for_kind<kBG1, kBG1Special>(learned, half, a, grid, threads, lds, s);
+    if (kind == kBG2) return slim_for_kind<kBG2, kBG2Special>(learned, half, a, grid, threads, lds, s);
+    return slim_for_kind<kWifi, kQCSpecial>(learned, half, a, grid, threads, lds, s);
 }
 
 int decoder_launch(int kind, bool learned, const DecodeArgs& a, hipStream_t s) {

@@ -265,7 +265,7 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
     static_assert(nd_columns_leading<KIND>(), "LDS-resident columns must be the leading ones");
     static_assert(!(MIXED && TRAIN), "mixed tiles use the plain/learned forward path");
     static_assert(!(HALF && (LEARNED || TRAIN || MIXED)), "the binary16 schedule is plain min-sum");
-    static_assert(!(SLIM && (TRAIN || HALF)), "the slim layout serves the plain and learned decodes");
+    static_assert(!(SLIM && TRAIN), "the slim layout serves the plain, learned and binary16 decodes");
     constexpr int ROWS = S::ROWS;
     constexpr int NND = S::NND;
     constexpr int ND1 = S::ND1 > 0 ? S::ND1 : 1;
@@ -622,7 +622,10 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                     if constexpr (j < NND) {
                         if (!MIXED || !dead) lds_st(addr[KC], cur[KC] + dtv[k]);
                     } else if constexpr (SLIM) {
-                        if (!MIXED || !dead) d1_st(S::T.d1_slot[j], xd1[k] + dtv[k]);
+                        float nv = xd1[k] + dtv[k];
+                        // HALF: stored in binary16 between iterations (the column's only update)
+                        if constexpr (HALF) nv = t + 1 < a.n_iter ? round_half(nv) : nv;
+                        if (!MIXED || !dead) d1_st(S::T.d1_slot[j], nv);
                     } else {
                         float& ap = app1[S::T.d1_slot[j]];
                         ap = ap + dtv[k];
@@ -765,12 +768,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode(DecodeArgs a) {
 // The slim layout with a persistent grid: every workgroup owns one slot of a.d1_scratch and
 // decodes blocks blockIdx.x, blockIdx.x + gridDim.x, ...  Register budget for three wavefronts
 // per SIMD (two 6-wavefront workgroups per CU, or one of 12).
-template <int KIND, int ZC, bool LEARNED>
+template <int KIND, int ZC, bool LEARNED, bool HALF = false>
 __global__ void __attribute__((amdgpu_flat_work_group_size(1, kSlimMaxBlock), amdgpu_waves_per_eu(3)))
 k_decode_slim(DecodeArgs a) {
     const int nblk = (a.n_cw + a.cpb - 1) / a.cpb;
     for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-        decode_block<KIND, ZC, LEARNED, false, false, false, true>(a, blk, nullptr);
+        decode_block<KIND, ZC, LEARNED, false, false, HALF, true>(a, blk, nullptr);
         __syncthreads();  // the next block's loads overwrite this block's LDS
     }
 }

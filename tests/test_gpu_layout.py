@@ -178,3 +178,35 @@ def test_packed_padding_bits_are_zero(nr, torch_gpu, kind, Z, m):
         print(f"case padding kind={kind} Z={Z} m={m} {name}", flush=True)
         last = decode(name)[:, -1]
         assert not (last & pad_mask).any(), f"{name}: padding bits set in {int(((last & pad_mask) != 0).sum())} codewords"
+
+
+@pytest.mark.parametrize("kind,Z,m", [(1, 384, 0), (1, 104, 12), (2, 208, 0), (2, 13, 0), (3, 96, 0)])
+def test_half_prc_slim_layout_matches_oracle(nr, oracle, torch_gpu, kind, Z, m):
+    """The HALF_PRC schedule in both layouts: binary16 storage of the degree-1 posteriors happens at
+    their single update (slim) instead of in registers after the row loop (dup); equal to the
+    oracle's HALF_PRC restatement, with and without early exit."""
+    torch = torch_gpu
+    code = nr.Code(kind, Z, m)
+    n = 2 * _cpb_max(nr, code, nr.LAYOUT_SLIM) + 3
+    llr, _ = _noisy_llr(nr, code, n, 1.0, RATE[kind], seed=3 * Z + m)
+    d_llr = torch.from_numpy(llr).cuda()
+    d_half = torch.empty((n, code.n_vn), dtype=torch.int16, device="cuda")
+    nr.llr_to_half(d_llr.data_ptr(), d_llr.numel(), d_half.data_ptr())
+    torch.cuda.synchronize()
+    half_llr = d_half.cpu().numpy().view(np.uint16)
+    ocode = oracle.code(ORC[kind], Z, m)
+    for layout, name in ((nr.LAYOUT_SLIM, "slim"), (nr.LAYOUT_DUP, "dup")):
+        code.set_launch_layout(layout, 0)
+        for early in (False, True):
+            print(f"case half layout={name} kind={kind} Z={Z} m={m} early={early}", flush=True)
+            d_hd = torch.zeros((n, code.hd_bytes), dtype=torch.uint8, device="cuda")
+            d_app = torch.zeros((n, code.n_vn), dtype=torch.float32, device="cuda")
+            d_it = torch.zeros(n, dtype=torch.int32, device="cuda")
+            code.decode_half(d_half.data_ptr(), n, 6, d_hd.data_ptr(), d_app.data_ptr(), d_it.data_ptr(),
+                             flags=nr.EARLY_EXIT if early else 0)
+            torch.cuda.synchronize()
+            hd = np.unpackbits(d_hd.cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
+            ohd, oapp, oit = oracle.decode_half(ocode, half_llr, 6, early_exit=early)
+            assert np.array_equal(hd, ohd)
+            assert np.array_equal(d_app.cpu().numpy().view(np.uint32), oapp.view(np.uint32))
+            assert np.array_equal(d_it.cpu().numpy(), oit)
