@@ -54,6 +54,7 @@ constexpr size_t kMaxLlrBytes = size_t(1) << 30;  // LLR buffer cap of a tuning 
 int autotune_layouts(ldpc_ctx* x, int n_iter, int max_candidates, int* cand_layout, int* cand_cpb,
                      int* cand_cw_per_cu, float* cand_round_us, int* n_candidates) {
     const int kind = x->code.kind, Z = x->code.Z;
+    if (x->cus < 1) { set_last_error("autotune: the device reports no compute units"); return LDPC_ENODEV; }
     NRLDPC_HIP(hipSetDevice(x->device));
     std::vector<Cand> cand;
     auto add = [&](int layout, int cpb) {

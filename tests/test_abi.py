@@ -96,3 +96,35 @@ def test_cpp_host_api_exported(nr):
             "nr_gpu_ldpc_load_compact(h_element const*, int, int, h_element const*, int, int, ldpc_params const*)"]
     missing = [w for w in want if w not in out]
     assert not missing, missing
+
+
+def test_launch_layout_calls_on_a_host_only_context(nr):
+    """Layout calls check their arguments on the CPU: the dup layout's counts are validated
+    against its workgroup and LDS limits, the slim layout and the autotuner need a device, and
+    the reported shape of a host-only context is the dup layout."""
+    code = nr.Code(2, 13, 0, -1)
+    cmax = 0
+    for c in range(512 // 13, 0, -1):  # the LDS limit is the device's, or a fallback without one
+        try:
+            code.set_launch_layout(nr.LAYOUT_DUP, c)
+            cmax = c
+            break
+        except nr.LdpcError:
+            pass
+    assert cmax >= 1
+    assert code.launch_layout(100) == (nr.LAYOUT_DUP, cmax, (100 + cmax - 1) // cmax)
+    for layout, cpb, status in ((nr.LAYOUT_DUP, cmax + 1, -1), (nr.LAYOUT_SLIM, 0, -4), (7, 0, -1),
+                                (nr.LAYOUT_AUTO, 3, -1)):
+        try:
+            code.set_launch_layout(layout, cpb)
+        except nr.LdpcError as e:
+            assert e.status == status, (layout, cpb, e.status)
+        else:
+            raise AssertionError(f"layout {layout} with {cpb} codewords per workgroup was accepted")
+    code.set_launch_layout(nr.LAYOUT_AUTO, 0)
+    try:
+        code.autotune(n_iter=4)
+    except nr.LdpcError as e:
+        assert e.status == -4
+    else:
+        raise AssertionError("autotune on a host-only context must fail")
