@@ -255,6 +255,13 @@ size_t ldpc_noms_workspace_bytes(const ldpc_ctx* ctx, int n_cw, int n_iter);
 int ldpc_noms_train_grad_async(ldpc_ctx* ctx, void* stream, const float* llr, const uint8_t* cw_bits,
                                int n_cw, int n_iter, float* grad, float* loss, void* workspace,
                                size_t workspace_bytes);
+/* ldpc_noms_train_grad_async with a loss weight per codeword: cw_weight (device, [n_cw], or NULL
+ * for all 1) multiplies each codeword's BCE terms in loss and gradient (the mean is still taken
+ * over n_cw codewords).  Used to weight the training frames, e.g. by their Eb/N0.  No reference
+ * counterpart. */
+int ldpc_noms_train_grad_weighted_async(ldpc_ctx* ctx, void* stream, const float* llr, const uint8_t* cw_bits,
+                                        const float* cw_weight, int n_cw, int n_iter, float* grad, float* loss,
+                                        void* ws, size_t ws_bytes);
 
 /* Adam update of the context's weights in place from grad [n_iter][nnz_full][2]; m and v are
  * caller-owned device moment buffers of the same size (zero before step 1). */

@@ -104,6 +104,8 @@ _sig("ldpc_noms_get_weights", ctypes.c_int, _P, _c_int_p, _P)
 _sig("ldpc_decode_learned_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_decode_learned_noalloc_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
 _sig("ldpc_noms_workspace_bytes", ctypes.c_size_t, _P, ctypes.c_int, ctypes.c_int)
+_sig("ldpc_noms_train_grad_weighted_async", ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P,
+     _P, _P, ctypes.c_size_t)
 _sig("ldpc_noms_train_grad_async", ctypes.c_int, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P,
      ctypes.c_size_t)
 _sig("ldpc_noms_adam_async", ctypes.c_int, _P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_float)
@@ -323,7 +325,14 @@ class Code:
     def workspace_bytes(self, n_cw, n_iter):
         return lib.ldpc_noms_workspace_bytes(self._h, n_cw, n_iter)
 
-    def train_grad(self, llr_ptr, bits_ptr, n_cw, n_iter, grad_ptr, loss_ptr, ws_ptr, ws_bytes, stream=0):
+    def train_grad(self, llr_ptr, bits_ptr, n_cw, n_iter, grad_ptr, loss_ptr, ws_ptr, ws_bytes, stream=0,
+                   weight_ptr=0):
+        """weight_ptr: device float [n_cw] loss weight per codeword (0: all 1)."""
+        if weight_ptr:
+            _check(lib.ldpc_noms_train_grad_weighted_async(self._h, stream, llr_ptr, bits_ptr, weight_ptr, n_cw,
+                                                           n_iter, grad_ptr, loss_ptr, ws_ptr, ws_bytes),
+                   "ldpc_noms_train_grad_weighted_async")
+            return
         _check(lib.ldpc_noms_train_grad_async(self._h, stream, llr_ptr, bits_ptr, n_cw, n_iter, grad_ptr,
                                               loss_ptr, ws_ptr, ws_bytes), "ldpc_noms_train_grad_async")
 

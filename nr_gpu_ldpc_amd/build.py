@@ -36,7 +36,8 @@ COMMON = [
     f"-I{CSRC}",
 ] + os.environ.get("NRLDPC_EXTRA_FLAGS", "").split()
 
-SOURCES = ["decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "decoder_pair.hip", "decoder_slim_bg1.hip", "decoder_slim_bg2.hip",
+SOURCES = ["mixed_k_dup_learned.hip", "mixed_k_slim_learned.hip", "decoder_pair.hip", "mixed_k_dup_plain.hip",
+           "mixed_k_slim_plain.hip", "decoder.hip", "decoder_bg1.hip", "decoder_bg2.hip", "decoder_qc.hip", "decoder_slim_bg1.hip", "decoder_slim_bg2.hip",
            "decoder_slim_qc.hip", "decode_tuning.cpp", "tune.cpp", "channel.hip", "learned.hip", "api.cpp", "decode.cpp", "code.cpp",
            "encoder.cpp", "noms_api.cpp", "compat.cpp", "mixed_plan.cpp", "mixed.hip",
            "tb_host.cpp", "tb.hip"]

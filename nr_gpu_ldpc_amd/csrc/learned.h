@@ -19,6 +19,7 @@ struct BackwardArgs {
     float* grad;           // [n_iter][nnz][2], accumulated
     float* loss;           // accumulated mean BCE
     float* gapp;           // workspace [n_wg][2][n_vn]
+    const float* cw_weight;  // [n_cw] weight of each codeword's loss terms, or null (all 1)
     int n_cw, n_iter, Z, m_rows, n_vn, nnz;
     float inv_norm;        // 1 / (n_vn * n_cw)
 };

@@ -3,7 +3,7 @@
 // Decoder (forward, k_decode<..., LEARNED>):  R_t(e) = s * alpha_t(e) * max(m - beta_t(e), 0)
 // with one (alpha, beta) per (iteration t, base-graph edge e); alpha = 0.75, beta = 0 is the
 // plain decoder.  Loss: mean binary cross-entropy of the soft output sigmoid(-APP_T) against
-// the transmitted bits, over every decoder column.
+// the transmitted bits, over every decoder column, each codeword's terms optionally weighted.
 //
 // Backward (this file), in exact-arithmetic form APP_t = L + sum_e R_t(e),
 // Q_t(e) = APP_{t-1}[c(e)] - R_{t-1}(e):
@@ -56,13 +56,15 @@ __global__ void __launch_bounds__(512) k_noms_backward(BackwardArgs a) {
     for (int cw = blockIdx.x; cw < a.n_cw; cw += gridDim.x) {
         const float* app = a.app + (size_t)cw * nvn;
         const uint8_t* bits = a.bits + (size_t)cw * nvn;
+        const float wcw = a.cw_weight ? a.cw_weight[cw] : 1.f;  // the codeword's loss weight
+        const float gnorm = wcw * a.inv_norm;
         // loss and dLoss/dAPP_T
         for (int n = r; n < nvn; n += blockDim.x) {
             const float x = app[n];
             const float sp = x > 0.f ? x + log1pf(expf(-x)) : log1pf(expf(x));  // softplus(x)
             const float sg = 1.f / (1.f + expf(-x));
-            loss_part += bits[n] ? sp : sp - x;
-            gcur[n] = (bits[n] ? sg : sg - 1.f) * a.inv_norm;
+            loss_part += wcw * (bits[n] ? sp : sp - x);
+            gcur[n] = (bits[n] ? sg : sg - 1.f) * gnorm;
         }
         float gq1[ROWS], gq2[ROWS];
         uint32_t pa1[ROWS], pa2[ROWS];

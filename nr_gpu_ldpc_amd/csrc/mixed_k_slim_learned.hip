@@ -1,0 +1,9 @@
+// mixed_k_slim_learned.hip -- instantiation of the persistent mixed-batch kernel, slim layout,
+// learned weights where set (with the plain fallback) (device code in mixed_kernel.h).
+#include "mixed_kernel.h"
+
+namespace nrldpc {
+
+const void* mixed_kernel_slim_learned() { return reinterpret_cast<const void*>(k_decode_mixed_slim<true>); }
+
+}  // namespace nrldpc

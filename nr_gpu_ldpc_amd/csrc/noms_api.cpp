@@ -247,6 +247,13 @@ size_t ldpc_noms_workspace_bytes(const ldpc_ctx* x, int n_cw, int n_iter) {
 
 int ldpc_noms_train_grad_async(ldpc_ctx* x, void* stream, const float* llr, const uint8_t* cw_bits, int n_cw,
                                int n_iter, float* grad, float* loss, void* ws, size_t ws_bytes) {
+    return ldpc_noms_train_grad_weighted_async(x, stream, llr, cw_bits, nullptr, n_cw, n_iter, grad, loss, ws,
+                                               ws_bytes);
+}
+
+int ldpc_noms_train_grad_weighted_async(ldpc_ctx* x, void* stream, const float* llr, const uint8_t* cw_bits,
+                                        const float* cw_weight, int n_cw, int n_iter, float* grad, float* loss,
+                                        void* ws, size_t ws_bytes) {
     if (!x || !llr || !cw_bits || !grad || !loss || !ws || n_cw <= 0) return LDPC_EINVAL;
     if (!x->d_weights || n_iter != x->weights_iters) {
         set_last_error("learned weights missing or for a different iteration count");
@@ -309,6 +316,7 @@ int ldpc_noms_train_grad_async(ldpc_ctx* x, void* stream, const float* llr, cons
     b.n_vn = c.n_vn();
     b.nnz = nnz_full(c.kind);
     b.inv_norm = 1.0f / ((float)c.n_vn() * (float)n_cw);
+    b.cw_weight = cw_weight;
     return backward_launch(c.kind, b, std::min(n_cw, 1024), static_cast<hipStream_t>(stream));
 }
 

@@ -76,8 +76,28 @@ def degree1_edges(kind, Z):
     return np.array([deg[j] == 1 for i in range(h.shape[0]) for j in range(h.shape[1]) if h[i, j] >= 0])
 
 
+def loss_weight_fn(spec):
+    """Per-frame loss weight as a function of the frame's Eb/N0 (dB), from a text spec:
+    "none" (all 1), "sigmoid:c,w" (1 / (1 + exp(-(e - c) / w)): frames well below the decoder's
+    threshold, whose BCE is large and whose bits no weight choice will fix, count little) or
+    "cutoff:c" (1 at e >= c, else 0)."""
+    if not spec or spec == "none":
+        return None
+    name, _, args = spec.partition(":")
+    vals = [float(v) for v in args.split(",") if v]
+    if name == "sigmoid" and len(vals) == 2:
+        c, w = vals
+        return lambda e: 1.0 / (1.0 + np.exp(-(np.asarray(e, np.float64) - c) / w))
+    if name == "cutoff" and len(vals) == 1:
+        return lambda e: (np.asarray(e, np.float64) >= vals[0]).astype(np.float64)
+    raise ValueError(f"unknown loss weight spec {spec!r}")
+
+
 def train(kind=2, Z=208, n_iter=10, codewords=1 << 20, batch=1024, lr=2e-3, ebn0=(-1.0, 3.0), seed=1,
-          device=0, log=print, freeze_degree1=False):
+          device=0, log=print, freeze_degree1=False, loss_weight=None, epochs=1):
+    """loss_weight: None, or a function of the frames' Eb/N0 array giving each frame's loss weight
+    (loss_weight_fn).  epochs: passes over the same `codewords` training codewords (the channel
+    regenerates them bit for bit from the seed, each with its own Eb/N0)."""
     import torch
     code = nr.Code(kind, Z, 0, device)
     code.set_weights(code.plain_weights(n_iter))
@@ -91,23 +111,30 @@ def train(kind=2, Z=208, n_iter=10, codewords=1 << 20, batch=1024, lr=2e-3, ebn0
     mom = torch.zeros(shape, dtype=torch.float32, device=dev)
     vel = torch.zeros(shape, dtype=torch.float32, device=dev)
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    wts = torch.ones(batch, dtype=torch.float32, device=dev)
     keep = None
     if freeze_degree1:
         keep = torch.from_numpy((~degree1_edges(kind, Z)).astype(np.float32)).to(dev).view(1, -1, 1)
     ws = torch.empty(code.workspace_bytes(batch, n_iter), dtype=torch.uint8, device=dev)
-    rng = np.random.default_rng(seed)
     rate = code_rate(code)
-    steps = max(1, codewords // batch)
+    per_epoch = max(1, codewords // batch)
+    steps = per_epoch * max(1, epochs)
     hist = []
     t0 = time.time()
+    rng = None
     for step in range(1, steps + 1):
-        sig.copy_(torch.from_numpy(sigmas_for(rng.uniform(ebn0[0], ebn0[1], batch), rate)))
-        code.channel(seed, (step - 1) * batch, batch, llr.data_ptr(), sigma_ptr=sig.data_ptr(),
+        if (step - 1) % per_epoch == 0:
+            rng = np.random.default_rng(seed)  # each epoch draws the same Eb/N0 sequence
+        eb = rng.uniform(ebn0[0], ebn0[1], batch)
+        sig.copy_(torch.from_numpy(sigmas_for(eb, rate)))
+        if loss_weight is not None:
+            wts.copy_(torch.from_numpy(np.asarray(loss_weight(eb), dtype=np.float32)))
+        code.channel(seed, ((step - 1) % per_epoch) * batch, batch, llr.data_ptr(), sigma_ptr=sig.data_ptr(),
                      cw_bits_ptr=bits.data_ptr(), stream=s)
         grad.zero_()
         loss.zero_()
         code.train_grad(llr.data_ptr(), bits.data_ptr(), batch, n_iter, grad.data_ptr(), loss.data_ptr(),
-                        ws.data_ptr(), ws.numel(), stream=s)
+                        ws.data_ptr(), ws.numel(), stream=s, weight_ptr=wts.data_ptr() if loss_weight else 0)
         if keep is not None:
             grad.mul_(keep)
         step_lr = lr * 0.5 * (1 + math.cos(math.pi * (step - 1) / steps))  # cosine decay
@@ -189,16 +216,21 @@ def main():
     p.add_argument("--early-exit", action="store_true", help="evaluate with syndrome early exit")
     p.add_argument("--freeze-degree1", action="store_true",
                    help="keep the weights of edges into degree-1 columns at their plain values")
+    p.add_argument("--epochs", type=int, default=1, help="passes over the same training codewords")
+    p.add_argument("--loss-weight", type=str, default="none",
+                   help='per-frame loss weight by Eb/N0: "none", "sigmoid:c,w" or "cutoff:c"')
     a = p.parse_args()
     path = a.out or weights_path(a.kind, a.Z, a.iters)
     if a.mode == "train":
         rng_db = tuple(float(x) for x in a.ebn0_train.split(","))
         code, w, hist, secs = train(a.kind, a.Z, a.iters, a.codewords, a.batch, a.lr, rng_db, a.seed,
-                                   freeze_degree1=a.freeze_degree1)
+                                   freeze_degree1=a.freeze_degree1, loss_weight=loss_weight_fn(a.loss_weight),
+                                   epochs=a.epochs)
         save_weights(path, a.kind, a.Z, a.iters, w, {"codewords": a.codewords, "batch": a.batch, "lr": a.lr,
                                                      "ebn0_train": list(rng_db), "final_loss": hist[-1][1],
                                                      "train_seconds": round(secs, 1), "seed": a.seed,
-                                                     "freeze_degree1": a.freeze_degree1})
+                                                     "freeze_degree1": a.freeze_degree1,
+                                                     "loss_weight": a.loss_weight, "epochs": a.epochs})
         print(json.dumps({"saved": os.path.relpath(path, ROOT), "loss": hist, "seconds": round(secs, 1)}))
     else:
         rec = load_weights(path)

@@ -439,6 +439,12 @@ void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_
 double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8_t *bits, int n_cw,
                              int n_iter, const double *alpha, const double *beta, double *grad_alpha,
                              double *grad_beta) {
+    return oracle_noms_loss_grad_weighted(c, llr, bits, NULL, n_cw, n_iter, alpha, beta, grad_alpha, grad_beta);
+}
+
+double oracle_noms_loss_grad_weighted(const oracle_code *c, const float *llr, const uint8_t *bits,
+                                      const double *cw_weight, int n_cw, int n_iter, const double *alpha,
+                                      const double *beta, double *grad_alpha, double *grad_beta) {
     /* Forward in the non-incremental form APP_t = L + sum_e R_t(e), Q_t(e) = APP_{t-1} - R_{t-1}(e),
      * which equals the incremental update in exact arithmetic.  Per row and iteration the
      * arg-min (idx1) and second arg-min (idx2) are kept for the backward pass. */
@@ -460,6 +466,7 @@ double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8
     for (int w = 0; w < n_cw; w++) {
         const float *L = llr + (size_t)w * nvn;
         const uint8_t *x = bits + (size_t)w * nvn;
+        const double wcw = cw_weight ? cw_weight[w] : 1.0; /* the codeword's loss weight */
         memset(R, 0, sizeof(double) * ER);
         for (int n = 0; n < nvn; n++) app[n] = L[n];
         for (int t = 1; t <= n_iter; t++) {
@@ -502,10 +509,10 @@ double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8
             const double a = app[n];
             const double sp_pos = (a > 0) ? a + log1p(exp(-a)) : log1p(exp(a)); /* softplus(a) */
             const double sp_neg = sp_pos - a;                                     /* softplus(-a) */
-            loss += x[n] ? sp_pos : sp_neg;
+            loss += wcw * (x[n] ? sp_pos : sp_neg);
             /* dloss/da = x*sigmoid(a) - (1-x)*sigmoid(-a) */
             const double sg = 1.0 / (1.0 + exp(-a));
-            gapp[n] = (x[n] ? sg : (sg - 1.0)) / ((double)nvn * n_cw);
+            gapp[n] = wcw * (x[n] ? sg : (sg - 1.0)) / ((double)nvn * n_cw);
         }
         if (!grad_alpha && !grad_beta) continue;
         memset(gQn, 0, sizeof(double) * ER);

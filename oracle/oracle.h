@@ -77,6 +77,11 @@ void oracle_decode_noms(const oracle_code *c, const float *llr, int n_cw, int n_
 double oracle_noms_loss_grad(const oracle_code *c, const float *llr, const uint8_t *bits,
                              int n_cw, int n_iter, const double *alpha, const double *beta,
                              double *grad_alpha, double *grad_beta);
+/* The same with a loss weight per codeword (cw_weight [n_cw], NULL = all 1): codeword w's BCE
+ * terms and their gradient are multiplied by cw_weight[w]; the mean is still over n_cw. */
+double oracle_noms_loss_grad_weighted(const oracle_code *c, const float *llr, const uint8_t *bits,
+                                      const double *cw_weight, int n_cw, int n_iter, const double *alpha,
+                                      const double *beta, double *grad_alpha, double *grad_beta);
 
 /* Syndrome weight of hard decisions hd ([cols*Z] bytes) for one codeword. */
 int oracle_syndrome(const oracle_code *c, const uint8_t *hd);

@@ -42,6 +42,9 @@ class Oracle:
         lib.oracle_noms_loss_grad.argtypes = [ctypes.POINTER(OracleCode), P, P, ctypes.c_int, ctypes.c_int,
                                               P, P, P, P]
         lib.oracle_noms_loss_grad.restype = ctypes.c_double
+        lib.oracle_noms_loss_grad_weighted.argtypes = [ctypes.POINTER(OracleCode), P, P, P, ctypes.c_int,
+                                                       ctypes.c_int, P, P, P, P]
+        lib.oracle_noms_loss_grad_weighted.restype = ctypes.c_double
         lib.oracle_syndrome.argtypes = [ctypes.POINTER(OracleCode), P]
         lib.oracle_noise_stddev.argtypes = [ctypes.c_float, ctypes.c_float]
         lib.oracle_noise_stddev.restype = ctypes.c_float
@@ -114,16 +117,25 @@ class Oracle:
                                     iters.ctypes.data, threads)
         return (hd, app, iters) if with_iters else (hd, app)
 
-    def noms_loss_grad(self, c, llr, bits, n_iter, alpha, beta, grad=True):
+    def noms_loss_grad(self, c, llr, bits, n_iter, alpha, beta, grad=True, cw_weight=None):
+        """Mean BCE of the learned decode and its gradient; cw_weight [n_cw] weights each codeword's
+        loss terms (None: the unweighted entry point)."""
         llr = np.ascontiguousarray(llr, dtype=np.float32)
         bits = np.ascontiguousarray(bits, dtype=np.uint8)
         alpha = np.ascontiguousarray(alpha, dtype=np.float64)
         beta = np.ascontiguousarray(beta, dtype=np.float64)
         ga = np.zeros_like(alpha) if grad else None
         gb = np.zeros_like(beta) if grad else None
-        loss = self.lib.oracle_noms_loss_grad(ctypes.byref(c), llr.ctypes.data, bits.ctypes.data, llr.shape[0],
-                                              n_iter, alpha.ctypes.data, beta.ctypes.data,
-                                              ga.ctypes.data if grad else None, gb.ctypes.data if grad else None)
+        gpa = ga.ctypes.data if grad else None
+        gpb = gb.ctypes.data if grad else None
+        if cw_weight is None:
+            loss = self.lib.oracle_noms_loss_grad(ctypes.byref(c), llr.ctypes.data, bits.ctypes.data, llr.shape[0],
+                                                  n_iter, alpha.ctypes.data, beta.ctypes.data, gpa, gpb)
+        else:
+            wt = np.ascontiguousarray(cw_weight, dtype=np.float64)
+            loss = self.lib.oracle_noms_loss_grad_weighted(ctypes.byref(c), llr.ctypes.data, bits.ctypes.data,
+                                                           wt.ctypes.data, llr.shape[0], n_iter, alpha.ctypes.data,
+                                                           beta.ctypes.data, gpa, gpb)
         return loss, ga, gb
 
     def syndrome(self, c, hd_row):

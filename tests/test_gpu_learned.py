@@ -52,11 +52,12 @@ def test_learned_matches_oracle(nr, oracle, torch_gpu, kind, Z):
     assert np.array_equal(app, oapp)
 
 
-@pytest.mark.parametrize("kind,Z,T", [(2, 13, 4), (3, 8, 3), (1, 16, 3)])
-def test_gradient_matches_oracle(nr, oracle, torch_gpu, kind, Z, T):
+@pytest.mark.parametrize("kind,Z,T,weighted", [(2, 13, 4, False), (3, 8, 3, False), (1, 16, 3, False),
+                                                (2, 13, 4, True)])
+def test_gradient_matches_oracle(nr, oracle, torch_gpu, kind, Z, T, weighted):
     """GPU backward vs the oracle's double-precision analytic gradient (itself checked
-    against finite differences on the CPU)."""
-    print(f"case gradient kind={kind} Z={Z} T={T}", flush=True)
+    against finite differences on the CPU), unweighted and with a loss weight per codeword."""
+    print(f"case gradient kind={kind} Z={Z} T={T} weighted={weighted}", flush=True)
     torch = torch_gpu
     code = nr.Code(kind, Z)
     rng = np.random.default_rng(7 + Z)
@@ -70,12 +71,15 @@ def test_gradient_matches_oracle(nr, oracle, torch_gpu, kind, Z, T):
     grad = torch.zeros((T, code.nnz_full, 2), dtype=torch.float32, device="cuda")
     loss = torch.zeros(1, dtype=torch.float32, device="cuda")
     ws = torch.empty(code.workspace_bytes(n_cw, T), dtype=torch.uint8, device="cuda")
+    cw_w = np.array([0.5, 2.0, 0.0, 1.0, 3.0, 0.25], dtype=np.float32) if weighted else None
+    d_w = torch.from_numpy(cw_w).cuda() if weighted else None
     code.train_grad(d_llr.data_ptr(), d_bits.data_ptr(), n_cw, T, grad.data_ptr(), loss.data_ptr(),
-                    ws.data_ptr(), ws.numel())
+                    ws.data_ptr(), ws.numel(), weight_ptr=d_w.data_ptr() if weighted else 0)
     torch.cuda.synchronize()
     oc = oracle.code(KIND[kind], Z)
     oloss, ga, gb = oracle.noms_loss_grad(oc, llr, bits, T, w[..., 0].reshape(-1).astype(np.float64),
-                                          w[..., 1].reshape(-1).astype(np.float64))
+                                          w[..., 1].reshape(-1).astype(np.float64),
+                                          cw_weight=None if cw_w is None else cw_w.astype(np.float64))
     g = grad.cpu().numpy()
     assert abs(float(loss.item()) - oloss) <= 1e-4 * abs(oloss) + 1e-6
     for got, want in ((g[..., 0].reshape(-1), ga), (g[..., 1].reshape(-1), gb)):

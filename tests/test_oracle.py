@@ -327,6 +327,43 @@ def test_noms_gradient_matches_finite_differences(oracle, nr):
     assert np.abs(ga).sum() > 0 and np.abs(gb).sum() > 0
 
 
+def test_weighted_noms_loss_is_the_weighted_sum_of_per_codeword_losses(oracle, nr):
+    """Per-codeword loss weights: the weighted loss and gradient of a batch equal the weighted
+    sum of each codeword's own (one-codeword) loss and gradient divided by the batch size; all-one
+    weights give the unweighted entry point's values; the weighted gradient matches central
+    finite differences of the weighted loss."""
+    code = oracle.code(OR_BG2, 3)
+    nnz = int((oracle.shifts(code) >= 0).sum())
+    T = 3
+    rng = np.random.default_rng(9)
+    c = nr.Code(2, 3, 0, -1)
+    bits = c.encode_host(rng.integers(0, 2, (3, c.K), dtype=np.uint8))
+    llr = ((1.0 - 2.0 * bits) * 1.2 + rng.standard_normal(bits.shape) * 1.1).astype(F)
+    llr[:, :6] = 0.0
+    alpha = rng.uniform(0.6, 0.9, T * nnz)
+    beta = rng.uniform(-0.2, 0.2, T * nnz)
+    w = np.array([0.25, 2.0, 0.0])
+    loss, ga, gb = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, cw_weight=w)
+    parts = [oracle.noms_loss_grad(code, llr[k:k + 1], bits[k:k + 1], T, alpha, beta) for k in range(3)]
+    assert np.isclose(loss, sum(w[k] * parts[k][0] for k in range(3)) / 3, rtol=1e-12, atol=0)
+    assert np.allclose(ga, sum(w[k] * parts[k][1] for k in range(3)) / 3, rtol=1e-10, atol=1e-15)
+    assert np.allclose(gb, sum(w[k] * parts[k][2] for k in range(3)) / 3, rtol=1e-10, atol=1e-15)
+    ones = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, cw_weight=np.ones(3))
+    plain = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta)
+    assert ones[0] == plain[0] and np.array_equal(ones[1], plain[1]) and np.array_equal(ones[2], plain[2])
+    h = 1e-6
+    for i in rng.choice(T * nnz, 6, replace=False):
+        for vec, g in ((alpha, ga), (beta, gb)):
+            v0 = vec[i]
+            vec[i] = v0 + h
+            lp, _, _ = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, grad=False, cw_weight=w)
+            vec[i] = v0 - h
+            lm, _, _ = oracle.noms_loss_grad(code, llr, bits, T, alpha, beta, grad=False, cw_weight=w)
+            vec[i] = v0
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[i]) <= 1e-5 + 1e-4 * abs(fd), (i, fd, g[i])
+
+
 @pytest.mark.parametrize("kind,Z,m", [(OR_WIFI, 5, 0), (OR_BG1, 3, 0), (OR_BG2, 4, 9)])
 def test_syndrome_is_dense_parity_check(oracle, kind, Z, m):
     """oracle_syndrome = weight of H x over GF(2) with H expanded densely (brute force)."""
