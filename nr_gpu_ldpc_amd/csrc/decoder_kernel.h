@@ -452,8 +452,13 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
         __syncthreads();
     };
 
+    // A wavefront none of whose lanes decodes a codeword still running (each one emitted at
+    // convergence, not live, or beyond the tile) leaves the iteration loop for the barrier-only
+    // loop below, which takes the workgroup's barriers and votes in the same order: its issue slots
+    // go to the wavefronts of the codewords still decoding.  Wave-uniform.
+    bool shadow = __builtin_amdgcn_readfirstlane((int)(idle || __ballot(live) == 0u)) != 0;
     int t = 0;
-    for (; t < a.n_iter; ++t) {
+    for (; t < a.n_iter && !shadow; ++t) {
         // Loop-variant copies of the lane bases keep address arithmetic from being hoisted
         // out of the iteration loop into long-lived VGPRs.
         asm volatile("" : "+v"(rRd), "+v"(rAcc), "+v"(rAccW), "+v"(rRdHi));
@@ -747,8 +752,50 @@ __device__ __forceinline__ bool decode_block(const DecodeArgs& a, const int blk,
                 // staging reuses the read copy of the converged codewords only
                 emit(fresh, t + 1);
                 if (fresh) emitted = true;
+                shadow = __builtin_amdgcn_readfirstlane((int)(__ballot(live && !emitted) == 0u)) != 0;
             }
-            if (!any_rest) break;
+            if (!any_rest) {
+                shadow = false;  // the whole workgroup is done
+                break;
+            }
+        }
+    }
+    if (shadow) {
+        // The rest of the workgroup's iterations, barriers and votes only (the loop above with
+        // every lane's work dropped: no codeword here is running, so it stores no syndrome flag,
+        // casts no vote and emits nothing).
+        for (; t < a.n_iter; ++t) {
+            static_for<0, ROWS>([&](auto IC) {
+                constexpr int i = IC;
+                if constexpr (S::T.epoch_start[i] != 0 && i != 0) {
+                    if (i < m_rows) __syncthreads();
+                }
+            });
+            __syncthreads();
+            const bool last = t + 1 == a.n_iter;
+            bool fin = false;
+            if constexpr (MIXED) fin = final_check && last;
+            else fin = a.fail_out != nullptr && last;
+            if (last && !fin) break;
+            const bool check = early || fin;
+            if (check && !uniform_cw && tid == 0) lds_stu(wgFlag, 0u);
+            __syncthreads();
+            if (check) {
+                __syncthreads();
+                if (fin) {
+                    final_ok = lds_ldu(flagAddr) == 0u;
+                    break;
+                }
+                bool any_fresh = false, any_rest = false;
+                if (!uniform_cw) {
+                    __syncthreads();
+                    const uint32_t m = lds_ldu(wgFlag);
+                    any_fresh = (m & 1u) != 0u;
+                    any_rest = (m & 2u) != 0u;
+                }
+                if (any_fresh) emit(false, t + 1);
+                if (!any_rest) break;
+            }
         }
     }
     bool out = live && !emitted;
