@@ -6,7 +6,7 @@
  * (NULL = the default stream).  Every *_async call only enqueues work on `stream`.
  * A context may be used from several host threads as long as calls on it are not
  * concurrent with ldpc_ctx_destroy / ldpc_set_launch / ldpc_set_launch_layout / ldpc_autotune /
- * ldpc_noms_set_weights.
+ * ldpc_set_layout_model / ldpc_noms_set_weights.
  *
  * Each entry point names the reference interface it replaces (paths relative to the
  * reference checkout).
@@ -152,6 +152,20 @@ int ldpc_launch_shape(const ldpc_ctx* ctx, int n_cw, int* cw_per_block, int* wor
  * fixes the grid at build time, src/setup_exp.cu:131-166). */
 int ldpc_autotune(ldpc_ctx* ctx, int n_iter, int max_candidates, int* cand_layout, int* cand_cpb,
                   int* cand_cw_per_cu, float* cand_round_us, int* n_candidates);
+
+/* The context's launch cost model of one layout (LDPC_LAYOUT_DUP or LDPC_LAYOUT_SLIM): the
+ * codewords per workgroup that layout launches with, the codewords per CU in one round and the
+ * round time in microseconds, as installed from the table built into the library, by
+ * ldpc_autotune or by ldpc_set_layout_model.  cw_per_block 0 means the layout is not in the model
+ * (the slim layout of a code the table has no slim entry for).  ldpc_set_layout_model installs a
+ * model measured earlier -- an autotune cache a caller keeps across processes -- without timing
+ * again, and returns to the per-batch choice (LDPC_LAYOUT_AUTO) as ldpc_autotune does; it refuses
+ * counts beyond the layout's workgroup or LDS limit, cw_per_cu < 1 and round times that are not
+ * positive and finite, and takes cw_per_block 0 only for the slim layout (which drops it from the
+ * model).  A set-up call: a slim model may allocate the slim layout's buffer.  No reference
+ * counterpart (the reference fixes the grid at build time, src/setup_exp.cu:131-166). */
+int ldpc_get_layout_model(const ldpc_ctx* ctx, int layout, int* cw_per_block, int* cw_per_cu, float* round_us);
+int ldpc_set_layout_model(ldpc_ctx* ctx, int layout, int cw_per_block, int cw_per_cu, float round_us);
 
 /* Repack per-codeword packed decisions ([n_cw][hd_bytes], as ldpc_decode_async writes them) into
  * one bit stream over the concatenated [n_cw][n_vn] columns: bit g of out (byte g/8, bit g%8) is

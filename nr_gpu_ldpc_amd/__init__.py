@@ -85,6 +85,8 @@ _sig("ldpc_launch_shape", ctypes.c_int, _P, ctypes.c_int, _P, _P)
 _sig("ldpc_set_launch_layout", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int)
 _sig("ldpc_launch_layout", ctypes.c_int, _P, ctypes.c_int, _P, _P, _P)
 _sig("ldpc_autotune", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P)
+_sig("ldpc_get_layout_model", ctypes.c_int, _P, ctypes.c_int, _P, _P, _P)
+_sig("ldpc_set_layout_model", ctypes.c_int, _P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float)
 _sig("ldpc_syndrome_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_hd_to_stream_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, _P)
 _sig("ldpc_decode_half_async", ctypes.c_int, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_uint, _P, _P, _P)
@@ -259,6 +261,20 @@ class Code:
         _check(lib.ldpc_autotune(self._h, int(n_iter), n, lay, cpb, per, rus, ctypes.byref(cnt)), "ldpc_autotune")
         return [{"layout": lay[k], "cpb": cpb[k], "cw_per_cu": per[k], "round_us": rus[k]}
                 for k in range(min(n, cnt.value))]
+
+    def layout_model(self, layout):
+        """The launch cost model of LAYOUT_DUP or LAYOUT_SLIM as a dict (cpb, cw_per_cu, round_us);
+        cpb 0: the layout is not in the model."""
+        cpb, per, rus = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_float(0.0)
+        _check(lib.ldpc_get_layout_model(self._h, int(layout), ctypes.byref(cpb), ctypes.byref(per),
+                                         ctypes.byref(rus)), "ldpc_get_layout_model")
+        return {"cpb": cpb.value, "cw_per_cu": per.value, "round_us": rus.value}
+
+    def set_layout_model(self, layout, cpb, cw_per_cu=0, round_us=0.0):
+        """Install a launch cost model measured earlier (an autotune cache); cpb 0 drops the slim
+        layout from the model."""
+        _check(lib.ldpc_set_layout_model(self._h, int(layout), int(cpb), int(cw_per_cu), float(round_us)),
+               "ldpc_set_layout_model")
 
     def decode(self, llr_ptr, n_cw, n_iter, hd_ptr, app_ptr=0, iters_ptr=0, flags=0, stream=0):
         _check(lib.ldpc_decode_async(self._h, stream, llr_ptr, n_cw, n_iter, flags, hd_ptr,

@@ -166,6 +166,7 @@ int ldpc_ctx_create(ldpc_ctx** out, int kind, int Z, int m_rows, int device) {
     if (st) { set_last_error(err); delete x; return st; }
     x->cpb_default = default_cpb(x->code);
     x->cpb = x->cpb_default;
+    x->m_dup.cpb = x->cpb_default;
     if (device < 0) {
         // host-only context: code description, encoder, generator tables; no device work
         x->device = -1;
@@ -301,6 +302,53 @@ int ldpc_autotune(ldpc_ctx* x, int n_iter, int max_candidates, int* cand_layout,
     if (x->device < 0) { set_last_error("host-only context"); return LDPC_ENODEV; }
     return autotune_layouts(x, n_iter, max_candidates, cand_layout, cand_cpb, cand_cw_per_cu, cand_round_us,
                             n_candidates);
+}
+
+int ldpc_get_layout_model(const ldpc_ctx* x, int layout, int* cpb, int* cw_per_cu, float* round_us) {
+    if (!x || (layout != LDPC_LAYOUT_DUP && layout != LDPC_LAYOUT_SLIM)) return LDPC_EINVAL;
+    const LayoutModel& m = layout == LDPC_LAYOUT_DUP ? x->m_dup : x->m_slim;
+    if (cpb) *cpb = m.cpb;
+    if (cw_per_cu) *cw_per_cu = m.cw_per_cu;
+    if (round_us) *round_us = m.round_us;
+    return LDPC_OK;
+}
+
+int ldpc_set_layout_model(ldpc_ctx* x, int layout, int cpb, int cw_per_cu, float round_us) {
+    if (!x) return LDPC_EINVAL;
+    const int kind = x->code.kind, Z = x->code.Z;
+    if (layout == LDPC_LAYOUT_SLIM && cpb == 0) {  // drop the slim layout from the model
+        x->m_slim = LayoutModel{};
+        x->layout_set = 0;
+        x->cpb = x->cpb_default;
+        return LDPC_OK;
+    }
+    if (layout != LDPC_LAYOUT_DUP && layout != LDPC_LAYOUT_SLIM) {
+        set_last_error("unknown layout");
+        return LDPC_EINVAL;
+    }
+    const int lim = layout == LDPC_LAYOUT_DUP ? decoder_dup_cpb_max(kind, Z) : decoder_slim_cpb_max(kind, Z);
+    if (cpb < 1 || cpb > lim) {
+        set_last_error("cw_per_block outside 1 .. the layout's workgroup or LDS limit");
+        return LDPC_EINVAL;
+    }
+    if (cw_per_cu < 1 || !(round_us > 0.f) || !std::isfinite(round_us)) {
+        set_last_error("cw_per_cu must be >= 1 and round_us positive and finite");
+        return LDPC_EINVAL;
+    }
+    const LayoutModel m{cpb, cw_per_cu, round_us};
+    if (layout == LDPC_LAYOUT_DUP) {
+        x->m_dup = m;
+        x->cpb_default = cpb;
+    } else {
+        if (x->device >= 0) {
+            const int st = ensure_slim(x, cpb);
+            if (st) return st;
+        }
+        x->m_slim = m;
+    }
+    x->cpb = x->cpb_default;
+    x->layout_set = 0;
+    return LDPC_OK;
 }
 
 int ldpc_decode_async(ldpc_ctx* x, void* stream, const float* llr, int n_cw, int n_iter, unsigned flags,

@@ -128,3 +128,38 @@ def test_launch_layout_calls_on_a_host_only_context(nr):
         assert e.status == -4
     else:
         raise AssertionError("autotune on a host-only context must fail")
+
+
+def test_layout_model_calls_on_a_host_only_context(nr):
+    """The launch cost model can be read and installed without a device (an autotune result kept
+    by the caller and restored in a later process): set/get round trip, the launch shape follows
+    the installed dup count, invalid models are refused and leave the model as it was, and the
+    slim layout can be dropped from the model and installed again."""
+    code = nr.Code(2, 13, 0, -1)
+    assert code.layout_model(nr.LAYOUT_DUP)["cpb"] >= 1
+    code.set_layout_model(nr.LAYOUT_DUP, 2, 7, 123.5)
+    assert code.layout_model(nr.LAYOUT_DUP) == {"cpb": 2, "cw_per_cu": 7, "round_us": 123.5}
+    assert code.launch_layout(100) == (nr.LAYOUT_DUP, 2, 50)
+    bad = [(nr.LAYOUT_DUP, 0, 1, 1.0), (nr.LAYOUT_DUP, 10 ** 6, 1, 1.0), (nr.LAYOUT_DUP, 1, 0, 1.0),
+           (nr.LAYOUT_DUP, 1, 1, 0.0), (nr.LAYOUT_DUP, 1, 1, -3.0), (nr.LAYOUT_DUP, 1, 1, float("nan")),
+           (nr.LAYOUT_DUP, 1, 1, float("inf")), (nr.LAYOUT_AUTO, 1, 1, 1.0), (7, 1, 1, 1.0),
+           (nr.LAYOUT_SLIM, -1, 1, 1.0), (nr.LAYOUT_SLIM, 10 ** 6, 1, 1.0)]
+    for args in bad:
+        try:
+            code.set_layout_model(*args)
+        except nr.LdpcError as e:
+            assert e.status == -1, (args, e.status)
+        else:
+            raise AssertionError(f"model {args} was accepted")
+    assert code.layout_model(nr.LAYOUT_DUP) == {"cpb": 2, "cw_per_cu": 7, "round_us": 123.5}
+    code.set_layout_model(nr.LAYOUT_SLIM, 0)
+    assert code.layout_model(nr.LAYOUT_SLIM) == {"cpb": 0, "cw_per_cu": 0, "round_us": 0.0}
+    code.set_layout_model(nr.LAYOUT_SLIM, 1, 3, 50.0)
+    assert code.layout_model(nr.LAYOUT_SLIM) == {"cpb": 1, "cw_per_cu": 3, "round_us": 50.0}
+    for layout in (0, 3):
+        try:
+            code.layout_model(layout)
+        except nr.LdpcError as e:
+            assert e.status == -1
+        else:
+            raise AssertionError(f"layout {layout} was accepted")

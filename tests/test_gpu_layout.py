@@ -126,6 +126,34 @@ def test_autotune_installs_a_shape_that_decodes_exactly(nr, oracle, torch_gpu, k
     assert np.array_equal(hd, ohd) and np.array_equal(app.view(np.uint32), oapp.view(np.uint32))
 
 
+def test_layout_model_restores_an_autotune_result(nr, oracle, torch_gpu):
+    """An autotune result read with ldpc_get_layout_model and installed into a fresh context with
+    ldpc_set_layout_model (an autotune cache across processes) gives the same per-batch launch
+    choices as the tuned context, and the restored slim shape decodes bit-exactly."""
+    print("case layout model restore BG2 Z=52", flush=True)
+    tuned = nr.Code(2, 52)
+    tuned.autotune(n_iter=4)
+    models = {lay: tuned.layout_model(lay) for lay in (nr.LAYOUT_DUP, nr.LAYOUT_SLIM)}
+    assert models[nr.LAYOUT_DUP]["cpb"] >= 1 and models[nr.LAYOUT_SLIM]["cpb"] >= 1
+    fresh = nr.Code(2, 52)
+    for lay, m in models.items():
+        fresh.set_layout_model(lay, m["cpb"], m["cw_per_cu"], m["round_us"])
+        assert fresh.layout_model(lay) == m
+    cus = torch_gpu.cuda.get_device_properties(0).multi_processor_count
+    for n in (1, 7, cus, 3 * cus + 5, 40 * cus):
+        assert fresh.launch_layout(n) == tuned.launch_layout(n), n
+    fresh.set_layout_model(nr.LAYOUT_SLIM, models[nr.LAYOUT_SLIM]["cpb"], 1, 1e-3)  # slim always cheaper
+    n = 3 * models[nr.LAYOUT_SLIM]["cpb"] + 1
+    assert fresh.launch_layout(n)[:2] == (nr.LAYOUT_SLIM, models[nr.LAYOUT_SLIM]["cpb"])
+    llr, _ = _noisy_llr(nr, fresh, n, 1.5, RATE[2], seed=5)
+    hd, app, its = _gpu_decode(torch_gpu, fresh, llr, 6, flags=nr.EARLY_EXIT)
+    ohd, oapp, oit = oracle.decode(oracle.code(ORC[2], 52), llr, 6, early_exit=True)
+    assert np.array_equal(its, oit)
+    assert np.array_equal(hd, ohd) and np.array_equal(app.view(np.uint32), oapp.view(np.uint32))
+    fresh.set_layout_model(nr.LAYOUT_SLIM, 0)
+    assert fresh.launch_layout(n)[0] == nr.LAYOUT_DUP
+
+
 def test_set_launch_selects_layout_by_count(nr, torch_gpu):
     """ldpc_set_launch: counts that fit the dup layout take it, larger ones the slim layout."""
     code = nr.Code(1, 384)
