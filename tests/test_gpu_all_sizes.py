@@ -57,6 +57,29 @@ def test_plain_every_lifting_size(nr, oracle, torch_gpu):
         code.close()
 
 
+def test_plain_slim_layout_every_lifting_size(nr, oracle, torch_gpu):
+    """The slim layout (single LDS read copy, degree-1 posteriors in device slots, persistent grid)
+    forced at every pair: one workgroup's codewords plus one, so that a second block runs through
+    the persistent loop, with early exit at an Eb/N0 where codewords stop at different
+    iterations."""
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 1)
+        code = nr.Code(kind, Z, m)
+        code.set_launch_layout(nr.LAYOUT_SLIM, 0)
+        cpb = code.launch_layout(1)[1]
+        n = min(cpb + 1, 400)
+        print(f"case plain slim kind={kind} Z={Z} m={m} cpb={cpb} n={n}", flush=True)
+        assert code.launch_layout(n)[:2] == (nr.LAYOUT_SLIM, cpb)
+        oc = oracle.code(KIND[kind], Z, m)
+        llr, _ = _noisy_llr(nr, code, n, 2.0, rate, seed=13 * Z + kind)
+        hd, app, it = _gpu_decode(torch_gpu, code, llr, 8, flags=nr.EARLY_EXIT)
+        ohd, oapp, oit = oracle.decode(oc, llr, 8, early_exit=True)
+        assert np.array_equal(it, oit), (kind, Z, m)
+        assert np.array_equal(hd, ohd), (kind, Z, m)
+        assert np.array_equal(app.view(np.uint32), oapp.view(np.uint32)), (kind, Z, m)
+        code.close()
+
+
 def test_half_every_lifting_size(nr, oracle, torch_gpu):
     for kind, Z in PAIRS:
         m, rate = _case(kind, Z, 1)
