@@ -1,9 +1,10 @@
 """Every TS 38.212 lifting size of both base graphs (102 (BG, Z) pairs) and the 12x24 code, decoded
 on the GPU by each schedule and compared with the oracle bit for bit: plain FP32 (with and without
 early exit, full and truncated parity rows), HALF_PRC, CHAR_PRC and learned offset/normalized
-min-sum with random per-(iteration, edge) weights; and the device encoder against the host one.
-Two or three noisy codewords per pair keep the oracle's share to a few seconds.  Each case prints
-its parameters before it runs."""
+min-sum with random per-(iteration, edge) weights, in the layout each batch gets and in the slim
+layout forced (one workgroup's codewords plus one); and the device encoder against the host one.
+A few noisy codewords per pair keep the oracle's share to a few seconds.  Each case prints its
+parameters before it runs."""
 import numpy as np
 import pytest
 
@@ -125,6 +126,35 @@ def test_learned_every_lifting_size(nr, oracle, torch_gpu):
         ohd, oapp = oracle.decode_noms(oracle.code(KIND[kind], Z, m), llr, T, wk[..., 0].reshape(-1),
                                        wk[..., 1].reshape(-1))
         assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
+        code.close()
+
+
+def test_half_and_learned_slim_layout_every_lifting_size(nr, oracle, torch_gpu):
+    """HALF_PRC and learned min-sum in the slim layout forced at every pair (a workgroup's
+    codewords plus one, capped), against the oracle's restatements."""
+    T = 3
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 2)
+        code = nr.Code(kind, Z, m)
+        code.set_launch_layout(nr.LAYOUT_SLIM, 0)
+        cpb = code.launch_layout(1)[1]
+        n = min(cpb + 1, 200)
+        print(f"case half+learned slim kind={kind} Z={Z} m={m} cpb={cpb} n={n}", flush=True)
+        assert code.launch_layout(n)[:2] == (nr.LAYOUT_SLIM, cpb)
+        oc = oracle.code(KIND[kind], Z, m)
+        llr, _ = _noisy_llr(nr, code, n, 1.5, rate, seed=19 * Z + kind)
+        llr_h = llr.astype(np.float16).view(np.uint16)
+        hd, app, _ = _half_decode(torch_gpu, code, llr_h, 4)
+        ohd, oapp, _ = oracle.decode_half(oc, llr_h, 4)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), ("half", kind, Z, m)
+        rng = np.random.default_rng(kind * 1000 + Z + 7)
+        w = np.stack([rng.uniform(0.5, 1.0, (T, code.nnz_full)), rng.uniform(-0.3, 0.5, (T, code.nnz_full))], -1)
+        w = w.astype(np.float32)
+        code.set_weights(w)
+        hd, app, _ = _learned(torch_gpu, code, llr, T)
+        wk = w[:, :code.nnz, :]
+        ohd, oapp = oracle.decode_noms(oc, llr, T, wk[..., 0].reshape(-1), wk[..., 1].reshape(-1))
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), ("learned", kind, Z, m)
         code.close()
 
 
