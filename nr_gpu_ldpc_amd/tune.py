@@ -11,7 +11,9 @@ independent) and keeps the fastest; the table is written as "kind Z cpb" lines t
 Single-code decodes (--decode): for every (base graph, Z) and the 12x24 code's sizes,
 ``ldpc_autotune`` times both LDS layouts (dup, slim) at each codewords-per-workgroup count on whole
 rounds of the device; the fastest count of each layout and its round time are written to
-csrc/decode_tuning.inc, the table compiled into the library (rebuild to use it).
+csrc/decode_tuning.inc, the table compiled into the library (rebuild to use it), and with
+--cache PATH also to a JSON cache that ``apply_cache`` installs into a context at run time
+(``ldpc_set_layout_model``), with no rebuild and no timing.
 
 A JSON summary (per candidate timings) goes to stdout.
 """
@@ -69,11 +71,12 @@ def best_of(cands, layout):
 
 def tune_decode(iters=10, device=0, log=print):
     shapes = [(k, z) for k in (nr.BG1, nr.BG2) for z in LIFTING_SIZES] + [(nr.QC12X24, z) for z in QC_SIZES]
-    table, lines = {}, []
+    table, lines, models = {}, [], {}
     t0 = time.time()
     for kind, Z in shapes:
         code = nr.Code(kind, Z, 0, device)
         cands = code.autotune(n_iter=iters)
+        models[f"{kind}/{Z}"] = layout_models(code)
         code.close()
         d, s = best_of(cands, nr.LAYOUT_DUP), best_of(cands, nr.LAYOUT_SLIM)
         table[f"{kind}/{Z}"] = cands
@@ -81,7 +84,38 @@ def tune_decode(iters=10, device=0, log=print):
         lines.append(f"{{{kind}, {Z}, {fmt(d)}, {fmt(s)}}},")
         log(f"kind {kind} Z {Z}: dup {d and (d['cpb'], round(d['round_us'] / d['cw_per_cu'], 3))} "
             f"slim {s and (s['cpb'], round(s['round_us'] / s['cw_per_cu'], 3))} us/cw ({time.time() - t0:.1f} s)")
-    return table, lines
+    return table, lines, models
+
+
+def layout_models(code):
+    """A context's launch cost model: {"dup": {cpb, cw_per_cu, round_us}, "slim": {...}}."""
+    return {"dup": code.layout_model(nr.LAYOUT_DUP), "slim": code.layout_model(nr.LAYOUT_SLIM)}
+
+
+def save_cache(path, models, device_name=""):
+    """Write {"kind/Z": layout_models(...)} as a JSON autotune cache."""
+    with open(path, "w") as f:
+        json.dump({"format": "nr_gpu_ldpc_amd layout models 1", "device": device_name, "codes": models}, f,
+                  indent=1, sort_keys=True)
+
+
+def apply_cache(code, path):
+    """Install the cached model of code's (kind, Z) into the context; False when the cache has none.
+    The slim layout is dropped from the model where the cache records none."""
+    with open(path) as f:
+        doc = json.load(f)
+    if doc.get("format") != "nr_gpu_ldpc_amd layout models 1":
+        raise ValueError(f"{path}: not a layout-model cache")
+    m = doc["codes"].get(f"{code.kind}/{code.Z}")
+    if m is None:
+        return False
+    d, s = m["dup"], m["slim"]
+    code.set_layout_model(nr.LAYOUT_DUP, d["cpb"], d["cw_per_cu"], d["round_us"])
+    if s["cpb"] > 0:
+        code.set_layout_model(nr.LAYOUT_SLIM, s["cpb"], s["cw_per_cu"], s["round_us"])
+    else:
+        code.set_layout_model(nr.LAYOUT_SLIM, 0)
+    return True
 
 
 def write_decode_table(lines, iters, path=DECODE_TABLE):
@@ -105,11 +139,15 @@ def main():
     p.add_argument("--out", default=DEFAULT_PATH)
     p.add_argument("--decode", action="store_true", help="tune the single-code decoder's launch layouts")
     p.add_argument("--table", default=DECODE_TABLE, help="--decode: where to write the table")
+    p.add_argument("--cache", default="", help="--decode: also write a JSON cache of the models (apply_cache)")
     a = p.parse_args()
     if a.decode:
         iters = a.iters or 10
-        table, lines = tune_decode(iters, log=lambda s: print(s, file=sys.stderr, flush=True))
+        table, lines, models = tune_decode(iters, log=lambda s: print(s, file=sys.stderr, flush=True))
         write_decode_table(lines, iters, a.table)
+        if a.cache:
+            import torch
+            save_cache(a.cache, models, torch.cuda.get_device_properties(0).name)
         print(json.dumps({"saved": os.path.relpath(a.table, ROOT), "iters": iters, "table": table}))
         return
     a.iters = a.iters or 20
