@@ -44,8 +44,13 @@ PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
 PEAK_VALU_GOPS = 256 * 64 * 2.4
 VALU_OPS_PER_EDGE = 8  # convention: sub, min, med3, sign, select, sign-apply, dt-sub, APP-add
 # counter summaries of the kernel that decodes the headline batch (the slim layout)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_decode_slim_bg1_z384_t10_b4096_traffic.json")
-ISSUE_FILE = os.path.join(ROOT, "profiles", "k_decode_slim_bg1_z384_t10_b4096_sq.json")
+# PMC summaries of the headline kernel, per LDS layout of the launch (slim: the per-batch choice at
+# 4096 codewords; dup: the layout of batches of at most one codeword per CU)
+PROFILE_STEM = {"slim": "k_decode_slim_bg1_z384_t10_b4096", "dup": "k_decode_bg1_z384_t10_b4096"}
+
+
+def profile_file(layout, kind):
+    return os.path.join(ROOT, "profiles", f"{PROFILE_STEM[layout]}_{kind}.json")
 # A CU issues at most one 64-lane VALU instruction per clock (4 SIMDs x 16 lanes).
 PEAK_VALU_ISSUE = 256 * 2.4e9
 STREAM_CLASSES = [1 << k for k in range(14)]  # combined-stream batch-size classes, 1 .. 8192
@@ -61,6 +66,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--skip-mixed", action="store_true")
     p.add_argument("--skip-stream", action="store_true")
+    p.add_argument("--layout", choices=("auto", "dup", "slim"), default="auto",
+                   help="LDS layout of the headline launch (auto: the per-batch choice of the tuned model)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 256 MiB device-to-device copy (scale factor for PMC byte counters)")
     return p.parse_args()
@@ -115,11 +122,11 @@ def cpu_baseline(llr_host, n_cw):
             "host": {"cpu": cpu_model(), "nproc": os.cpu_count()}}
 
 
-def traffic_per_launch():
+def traffic_per_launch(layout):
     """HBM bytes per decode launch measured by rocprofv3 PMC counters (FETCH_SIZE + WRITE_SIZE,
     each scaled by the counter's reading for a copy of known size), if profiled."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(profile_file(layout, "traffic")) as f:
             t = json.load(f)
         return t["bytes_per_launch"], t
     except (OSError, ValueError, KeyError):
@@ -128,6 +135,9 @@ def traffic_per_launch():
 
 def bench_headline(torch, nr, args):
     code = nr.Code(nr.BG1, Z)
+    if args.layout != "auto":
+        code.set_launch_layout(nr.LAYOUT_DUP if args.layout == "dup" else nr.LAYOUT_SLIM, 0)
+    layout = "slim" if code.launch_layout(BATCH)[0] == nr.LAYOUT_SLIM else "dup"
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     s = stream.cuda_stream
@@ -162,9 +172,9 @@ def bench_headline(torch, nr, args):
     alg_bytes = BATCH * (4 * code.n_vn + code.hd_bytes)
     valu_gops = valu_ops / (ms * 1e6)
     hbm_gbs = alg_bytes / (ms * 1e6)
-    traffic, tsrc = traffic_per_launch()
+    traffic, tsrc = traffic_per_launch(layout)
     try:
-        with open(ISSUE_FILE) as f:
+        with open(profile_file(layout, "sq")) as f:
             issue = json.load(f)
         n_valu = issue["valu_insts_per_launch"]
         issued = {"valu_insts_per_launch": n_valu, "lane_ops_per_edge_iteration": round(n_valu * 64 / (BATCH * edges * N_ITER), 2),
@@ -201,7 +211,7 @@ def bench_headline(torch, nr, args):
     char_fe = int(counts.tolist()[1]) - frame_err
     res = {
         "value": round(gbps, 4), "ms_per_step": round(ms, 4),
-        "launch_shape": dict(zip(("cw_per_block", "workgroups"), code.launch_shape(BATCH))),
+        "launch_shape": dict(zip(("cw_per_block", "workgroups"), code.launch_shape(BATCH)), layout=layout),
         "latency_ms": {"p50": round(statistics.median(per), 4), "p99": round(pct(per, 0.99), 4),
                        "kind": "device time per batch of 4096"},
         "fer": round(frame_err / BATCH, 6), "ber": bit_err / (BATCH * code.K),
