@@ -19,9 +19,10 @@
 //   types                             src/types.h:16-142
 //
 // Differences, all deliberate:
-//   * T = float (FP32 decode) and T = char (the reference's default CHAR_PRC build: LLRs
-//     quantised by quantize(0.5, x, 8), decoded by ldpc_decode_char_async) are instantiated;
-//     half is not (the binary16 schedule is ldpc_decode_half_async in the C ABI).
+//   * T = float (FP32 decode), T = half (the reference's HALF_PRC build: LLRs rounded to binary16
+//     by __float2half_rn as its reduce_precision<half> does, decoded by ldpc_decode_half_async)
+//     and T = char (its default CHAR_PRC build: LLRs quantised by quantize(0.5, x, 8), decoded by
+//     ldpc_decode_char_async) are instantiated.
 //   * dev_mem::dt and dev_mem::r stay NULL: the decoder keeps its check-node state on chip.
 //   * Errors: HIP failures print the library's message and exit(EXIT_FAILURE), as the
 //     reference's checkCudaErrors does; invalid configurations print and abort().
@@ -38,6 +39,7 @@
 #ifndef NRLDPC_NR_GPU_LDPC_HPP
 #define NRLDPC_NR_GPU_LDPC_HPP
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include "ldpc_api.h"
@@ -233,6 +235,11 @@ extern template dev_mem<float> setup_dev_memory<float>(const params*, const ldpc
 extern template void generate_new_codewords<float>(const params*, host_mem<float>*, const ldpc_params*, int, float);
 extern template void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, const cuda_grid*, int,
                                                       int);
+extern template host_mem<__half> setup_host<__half>(const params*, unsigned int, unsigned int);
+extern template dev_mem<__half> setup_dev_memory<__half>(const params*, const ldpc_params*, const host_mem<__half>*);
+extern template void generate_new_codewords<__half>(const params*, host_mem<__half>*, const ldpc_params*, int, float);
+extern template void perform_gpu_ldpc_decoding<__half>(dev_mem<__half>*, host_mem<__half>*, const cuda_grid*, int,
+                                                       int);
 extern template host_mem<char> setup_host<char>(const params*, unsigned int, unsigned int);
 extern template dev_mem<char> setup_dev_memory<char>(const params*, const ldpc_params*, const host_mem<char>*);
 extern template void generate_new_codewords<char>(const params*, host_mem<char>*, const ldpc_params*, int, float);

@@ -277,8 +277,8 @@ void unpack_char(char* packed_input, char* unpacked_output, int size) {
 
 template <typename T>
 host_mem<T> setup_host(const params* p, unsigned int n_info_bits, unsigned int n_cw_bits) {
-    static_assert(std::is_same<T, float>::value || std::is_same<T, char>::value,
-                  "the FP32 and CHAR_PRC decode paths are instantiated");
+    static_assert(std::is_same<T, float>::value || std::is_same<T, __half>::value || std::is_same<T, char>::value,
+                  "the FP32, HALF_PRC and CHAR_PRC decode paths are instantiated");
     host_mem<T> h{};
     h.mem_size_infobits = n_info_bits * sizeof(int);
     h.mem_size_cw = n_cw_bits * sizeof(int);
@@ -358,9 +358,11 @@ void generate_new_codewords(const params* p, host_mem<T>* h, const ldpc_params* 
         transmit(h->info_bin, h->cwds, h->modulated_cwds, h->mem_size_cw, h->recv, l->K, nb, sigma, l->bg_index);
         (void)error_check(h->modulated_cwds, h->recv, nb);
         calculate_llr(h->llr, h->recv, nb, sigma, z_punct);
-        // reduce_precision (src/setup_exp.cu:26-46): chars as quantize(0.5, x, 8) makes them
+        // reduce_precision (src/setup_exp.cu:26-46): chars as quantize(0.5, x, 8) makes them,
+        // halves rounded to nearest even
         for (int j = 0; j < nb; j++) {
             if constexpr (std::is_same<T, char>::value) h->llr_quan[j] = (char)quantize(0.5, h->llr[j], 8);
+            else if constexpr (std::is_same<T, __half>::value) h->llr_quan[j] = __float2half_rn(h->llr[j]);
             else h->llr_quan[j] = (T)h->llr[j];
         }
         // the same codeword goes to every stream (src/setup_exp.cu:187-193)
@@ -480,6 +482,9 @@ void perform_gpu_ldpc_decoding(dev_mem<T>* d, host_mem<T>* h, const cuda_grid* g
     if constexpr (std::is_same<T, char>::value)  // the CHAR_PRC build
         NRLDPC_CHECK(ldpc_decode_char_async(g_ctx, st, reinterpret_cast<const int8_t*>(d->llr[stream_index]), n_cw,
                                             n_iterations, 0, per_cw, nullptr, nullptr));
+    else if constexpr (std::is_same<T, __half>::value)  // the HALF_PRC build
+        NRLDPC_CHECK(ldpc_decode_half_async(g_ctx, st, reinterpret_cast<const uint16_t*>(d->llr[stream_index]), n_cw,
+                                            n_iterations, 0, per_cw, nullptr, nullptr));
     else
         NRLDPC_CHECK(ldpc_decode_async(g_ctx, st, reinterpret_cast<const float*>(d->llr[stream_index]), n_cw,
                                        n_iterations, 0, per_cw, nullptr, nullptr));
@@ -515,6 +520,10 @@ template host_mem<float> setup_host<float>(const params*, unsigned int, unsigned
 template dev_mem<float> setup_dev_memory<float>(const params*, const ldpc_params*, const host_mem<float>*);
 template void generate_new_codewords<float>(const params*, host_mem<float>*, const ldpc_params*, int, float);
 template void perform_gpu_ldpc_decoding<float>(dev_mem<float>*, host_mem<float>*, const cuda_grid*, int, int);
+template host_mem<__half> setup_host<__half>(const params*, unsigned int, unsigned int);
+template dev_mem<__half> setup_dev_memory<__half>(const params*, const ldpc_params*, const host_mem<__half>*);
+template void generate_new_codewords<__half>(const params*, host_mem<__half>*, const ldpc_params*, int, float);
+template void perform_gpu_ldpc_decoding<__half>(dev_mem<__half>*, host_mem<__half>*, const cuda_grid*, int, int);
 template host_mem<char> setup_host<char>(const params*, unsigned int, unsigned int);
 template dev_mem<char> setup_dev_memory<char>(const params*, const ldpc_params*, const host_mem<char>*);
 template void generate_new_codewords<char>(const params*, host_mem<char>*, const ldpc_params*, int, float);

@@ -8,11 +8,13 @@
 //
 //   ldpc_ber [--code nr|qc] [--info-bits A] [--rate R] [--Z Z] [--iters T] [--snr-min S0]
 //            [--snr-max S1] [--snr-step dS] [--codewords N] [--cw-per-mcw C] [--mcw M]
-//            [--streams S] [--seed X] [--dump PREFIX] [--prec float|char]
+//            [--streams S] [--seed X] [--dump PREFIX] [--prec float|half|char]
 //
 // --prec char runs the reference's default CHAR_PRC build (LLRs quantised by quantize(0.5, x,
-// 8), the signed-char schedule); float (the default here) the FP32 one.
-// --dump PREFIX writes the first batch's LLRs (float32, or int8 with --prec char,
+// 8), the signed-char schedule), --prec half its HALF_PRC build (binary16 LLRs and storage);
+// float (the default here) the FP32 one.
+// --dump PREFIX writes the first batch's LLRs (float32, binary16 bit patterns with --prec half,
+// or int8 with --prec char,
 // [n_cw][N_before_puncture]) and packed hard decisions of stream 0 to PREFIX.llr / PREFIX.hd,
 // for checking against the oracle.
 #include <chrono>
@@ -134,7 +136,7 @@ bool parse(int argc, char** argv, Options* o) {
         else return false;
     }
     return o->snr_step > 0 && o->cw_per_mcw > 0 && o->mcw > 0 && o->streams > 0 && o->iters > 0 &&
-           (o->prec == "float" || o->prec == "char");
+           (o->prec == "float" || o->prec == "half" || o->prec == "char");
 }
 
 // T = float: the FP32 build; T = char: the reference's default CHAR_PRC build.
@@ -162,7 +164,7 @@ int main(int argc, char** argv) {
     if (!parse(argc, argv, &o)) {
         fprintf(stderr, "usage: ldpc_ber [--code nr|qc] [--info-bits A] [--rate R] [--Z Z] [--iters T] "
                         "[--snr-min S] [--snr-max S] [--snr-step S] [--codewords N] [--cw-per-mcw C] [--mcw M] "
-                        "[--streams S] [--seed X] [--dump PREFIX] [--prec float|char]\n");
+                        "[--streams S] [--seed X] [--dump PREFIX] [--prec float|half|char]\n");
         return 2;
     }
     params p = {o.snr_min, o.snr_max, o.snr_step, o.codewords, o.info_bits, o.rate, o.cw_per_mcw, o.mcw, o.streams};
@@ -172,6 +174,7 @@ int main(int argc, char** argv) {
     fprintf(stderr, "code bg=%d Z=%d K=%d N=%d rows=%d cols=%d iters=%d streams=%d batch=%dx%d\n", ldpc.bg_index,
             ldpc.Z, ldpc.K, ldpc.N, ldpc.H_rows, ldpc.H_cols, ldpc.n_iterations, p.n_streams, p.n_mcw, p.n_cw_per_mcw);
     if (o.prec == "char") run<char>(ldpc, p, o);
+    else if (o.prec == "half") run<__half>(ldpc, p, o);
     else run<float>(ldpc, p, o);
     return 0;
 }

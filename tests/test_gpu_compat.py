@@ -72,6 +72,37 @@ def test_driver_char_build_qc_z96_sweep(oracle, tmp_path):
     assert oor == 0
 
 
+def test_driver_half_build_qc_z96_sweep(oracle, tmp_path):
+    """The reference's HALF_PRC build through the same driver (T = half): LLRs rounded to binary16
+    as its reduce_precision<half> does (__float2half_rn, src/setup_exp.cu:33-38), decoded bit for
+    bit as the oracle's HALF_PRC restatement decodes them."""
+    print("case driver HALF_PRC QC Z=96 SNR 0..10", flush=True)
+    args = ["--code", "qc", "--Z", "96", "--iters", "5", "--snr-min", "0", "--snr-max", "10", "--snr-step", "1",
+            "--codewords", "80", "--cw-per-mcw", "8", "--mcw", "10", "--streams", "8", "--prec", "half"]
+    lines, llr_h, hd = _run(args, tmp_path, dtype=np.uint16)
+    assert len(lines) == 11 and all(x["frames"] == 640 for x in lines)
+    fer = [x["fer"] for x in lines]
+    assert fer[0] > 0.5 and fer[-1] == 0.0
+    n_vn = 24 * 96
+    ohd, _, _ = oracle.decode_half(oracle.code(OR_WIFI, 96), llr_h.reshape(-1, n_vn), 5)
+    want = np.packbits(ohd.reshape(-1), bitorder="little")
+    assert np.array_equal(hd[:want.size], want)
+
+
+def test_driver_half_build_bg1_z384(oracle, tmp_path):
+    """HALF_PRC build on the reference's shipped main() code (BG1 from setup_ldpc(8424, 1/3))."""
+    print("case driver HALF_PRC BG1 Z=384", flush=True)
+    args = ["--info-bits", "8424", "--rate", "0.3333333", "--iters", "5", "--snr-min", "1.0", "--snr-max", "1.5",
+            "--snr-step", "0.5", "--codewords", "8", "--cw-per-mcw", "2", "--mcw", "4", "--streams", "2",
+            "--prec", "half"]
+    lines, llr_h, hd = _run(args, tmp_path, dtype=np.uint16)
+    assert all(x["frames"] == 16 for x in lines)
+    n_vn = 68 * 384
+    ohd, _, _ = oracle.decode_half(oracle.code(OR_BG1, 384), llr_h.reshape(-1, n_vn), 5)
+    want = np.packbits(ohd.reshape(-1), bitorder="little")
+    assert np.array_equal(hd[:want.size], want)
+
+
 def test_stream_packing_for_odd_lifting_size(nr, torch_gpu):
     """Per-codeword packed decisions -> one bit stream over [cw][n_vn] (n_vn % 8 != 0)."""
     torch = torch_gpu
