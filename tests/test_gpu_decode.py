@@ -149,6 +149,34 @@ def test_special_values(nr, oracle, torch_gpu):
     assert np.array_equal(app.view(np.uint32)[nz], oapp.view(np.uint32)[nz])
 
 
+@pytest.mark.parametrize("kind,Z", [(1, 384), (2, 13)])
+def test_special_values_slim_layout(nr, oracle, torch_gpu, kind, Z):
+    """The special values above in the slim layout (single read copy with wrap-selected reads,
+    degree-1 posteriors in device slots), with compiled-in offsets (BG1 Z=384) and the per-edge
+    table (BG2 Z=13), through the persistent loop (more codewords than one workgroup holds)."""
+    print(f"case special values slim kind={kind} Z={Z}", flush=True)
+    code = nr.Code(kind, Z)
+    code.set_launch_layout(nr.LAYOUT_SLIM, 0)
+    cpb = code.launch_layout(1)[1]
+    rng = np.random.default_rng(5 + Z)
+    n = code.n_vn
+    base = [np.full(n, 2000.0, np.float32),
+            np.where(rng.random(n) < 0.5, -1.0, 1.0).astype(np.float32),
+            np.where(rng.random(n) < 0.5, -0.0, 0.0).astype(np.float32),
+            (rng.standard_normal(n) * 3000).astype(np.float32),
+            np.round(rng.standard_normal(n) * 3).astype(np.float32)]
+    n_cw = max(len(base), cpb + 1)
+    llr = np.stack([base[i % len(base)] for i in range(n_cw)])
+    llr[:, :code.punct * Z] = 0.0
+    assert code.launch_layout(n_cw)[0] == nr.LAYOUT_SLIM
+    hd, app, _ = _gpu_decode(torch_gpu, code, llr, 6)
+    ohd, oapp, _ = oracle.decode(oracle.code(KIND[kind], Z), llr, 6)
+    assert np.array_equal(hd, ohd)
+    assert np.array_equal(app, oapp)
+    nz = oapp != 0
+    assert np.array_equal(app.view(np.uint32)[nz], oapp.view(np.uint32)[nz])
+
+
 def test_noiseless_roundtrip_full_batch(nr, torch_gpu):
     """BG1 Z=384, 4096 codewords at the bench size: noiseless codewords decode to themselves."""
     print("case BG1 Z=384 4096 noiseless", flush=True)
