@@ -52,3 +52,30 @@ def test_half_early_exit_and_conversion(nr, oracle, torch_gpu):
     ohd, oapp, oit = oracle.decode_half(oracle.code(OR_BG1, 64), llr_h, 20, early_exit=True)
     assert np.array_equal(it, oit) and it.min() < 20
     assert np.array_equal(hd, ohd) and np.array_equal(app, oapp)
+
+
+@pytest.mark.parametrize("kind,Z,layout_name", [(1, 384, "slim"), (1, 384, "dup"), (2, 13, "slim"), (3, 8, "dup")])
+def test_half_special_values(nr, oracle, torch_gpu, kind, Z, layout_name):
+    """binary16 edge cases as channel LLRs: signed zeros, exact ties, subnormals, values near the
+    largest finite binary16 (65504) and their mixture, in both LDS layouts; decisions and
+    posteriors equal the oracle's HALF_PRC restatement (posteriors up to the sign of an exactly
+    zero value, which the decoder canonicalises)."""
+    torch = torch_gpu
+    print(f"case half special values kind={kind} Z={Z} layout={layout_name}", flush=True)
+    code = nr.Code(kind, Z)
+    code.set_launch_layout(nr.LAYOUT_SLIM if layout_name == "slim" else nr.LAYOUT_DUP, 0)
+    rng = np.random.default_rng(Z + kind)
+    n = code.n_vn
+    sign = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    rows = [np.full(n, 60000.0), sign * 1.0, sign * 0.0, sign * 6e-8 * rng.integers(1, 1000, n),
+            sign * rng.choice([0.0, 6e-6, 1.0, 3.0, 3e4, 65504.0], n), rng.standard_normal(n) * 2000]
+    llr = np.stack(rows).astype(np.float16)
+    llr[:, :code.punct * Z] = 0
+    llr_h = llr.view(np.uint16)
+    assert (np.abs(llr[3]) < 6.2e-5).all() and (np.abs(llr[3]) > 0).any()  # subnormal binary16
+    hd, app, _ = _half_decode(torch, code, llr_h, 6)
+    ohd, oapp, _ = oracle.decode_half(oracle.code(KIND[kind], Z), llr_h, 6)
+    assert np.array_equal(hd, ohd)
+    assert np.array_equal(app, oapp)
+    nz = oapp != 0
+    assert np.array_equal(app.view(np.uint32)[nz], oapp.view(np.uint32)[nz])
