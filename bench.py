@@ -690,6 +690,42 @@ def bench_transport_block(torch, nr, args):
             "code_blocks": t.C, "code_blocks_passed": passed, "payload_recovered": ok_tb}
 
 
+def bench_small_batches(torch, nr, args):
+    """Device time of single-code decodes of small batches (BG1 Z=384 and BG2 Z=208, T=10 fixed,
+    Eb/N0 1.5 dB): a batch below one codeword per CU still takes one round of the fused kernel,
+    which bounds the latency of tiny batches."""
+    out = {"workload": "single-code decodes of 1, 16 and 256 codewords, T=10 fixed iterations, LLRs "
+                       "resident in HBM; device time per launch (event pair on the decode stream)",
+           "kind": "device time per batch"}
+    stream = torch.cuda.current_stream()
+    s = stream.cuda_stream
+    for kind, z, rate in ((nr.BG1, 384, 1 / 3), (nr.BG2, 208, 1 / 5)):
+        code = nr.Code(kind, z)
+        row = {}
+        for n in (1, 16, 256):
+            llr = torch.empty((n, code.n_vn), dtype=torch.float32, device="cuda")
+            hd = torch.zeros((n, code.hd_bytes), dtype=torch.uint8, device="cuda")
+            code.channel(77, 0, n, llr.data_ptr(), sigma=nr.noise_stddev(1.5, rate), stream=s)
+            for _ in range(3):
+                code.decode(llr.data_ptr(), n, N_ITER, hd.data_ptr(), stream=s)
+            reps = 50
+            ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+            for a, b_ in ev:
+                a.record(stream)
+                code.decode(llr.data_ptr(), n, N_ITER, hd.data_ptr(), stream=s)
+                b_.record(stream)
+            torch.cuda.synchronize()
+            lat = [a.elapsed_time(b_) for a, b_ in ev]
+            lay, cpb, wg = code.launch_layout(n)
+            row[str(n)] = {"latency_ms": {"p50": round(statistics.median(lat), 4), "p99": round(pct(lat, 0.99), 4)},
+                           "info_gbps": round(n * code.K / (statistics.median(lat) * 1e6), 4),
+                           "layout": "slim" if lay == nr.LAYOUT_SLIM else "dup", "cw_per_block": cpb,
+                           "workgroups": wg}
+        out[f"bg{kind}_z{z}"] = row
+        code.close()
+    return out
+
+
 def bench_operating_point(torch, nr, args):
     """Info throughput at the FER 1e-3 operating point of each decoder (BG1 Z=384, T <= 10 with
     early exit, batches of 4096): plain min-sum at its FER 1e-3 Eb/N0, the learned decoder (with
@@ -856,6 +892,7 @@ def main():
     if op:
         out["fer1e-3_operating_point"] = op
     out["ebn0_sweep"] = bench_ebn0_sweep(torch, nr, args)
+    out["small_batches"] = bench_small_batches(torch, nr, args)
     if not args.skip_mixed:
         out["mixed_batch"] = bench_mixed(torch, nr, args)
         out["transport_block"] = bench_transport_block(torch, nr, args)
