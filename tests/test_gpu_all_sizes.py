@@ -1,6 +1,7 @@
 """Every TS 38.212 lifting size of both base graphs (102 (BG, Z) pairs) and the 12x24 code, decoded
 on the GPU by each schedule and compared with the oracle bit for bit: plain FP32 (with and without
-early exit, full and truncated parity rows), HALF_PRC, CHAR_PRC and learned offset/normalized
+early exit, full and truncated parity rows), HALF_PRC and CHAR_PRC (each with and without early
+exit), learned offset/normalized
 min-sum with random per-(iteration, edge) weights, in the layout each batch gets and in the slim
 layout forced (one workgroup's codewords plus one); and the device encoder against the host one.
 A few noisy codewords per pair keep the oracle's share to a few seconds.  Each case prints its
@@ -106,6 +107,34 @@ def test_char_every_lifting_size(nr, oracle, torch_gpu):
         ohd, oapp, _, _ = oracle.decode_char(oracle.code(KIND[kind], Z, m), q, 4)
         assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), (kind, Z, m)
         code.close()
+
+
+def test_half_and_char_early_exit_every_lifting_size(nr, oracle, torch_gpu):
+    """HALF_PRC and CHAR_PRC with early exit at every pair: five codewords at an Eb/N0 where they
+    stop at different iterations (an odd count leaves the CHAR pair kernel's last codeword without
+    a partner); iteration counts, decisions and posteriors equal the oracle's restatements."""
+    stopped = {"half": 0, "char": 0}
+    for kind, Z in PAIRS:
+        m, rate = _case(kind, Z, 4)
+        print(f"case half+char early exit kind={kind} Z={Z} m={m}", flush=True)
+        code = nr.Code(kind, Z, m)
+        oc = oracle.code(KIND[kind], Z, m)
+        llr, _ = _noisy_llr(nr, code, 5, 3.0, rate, seed=23 * Z + kind)
+        llr_h = llr.astype(np.float16).view(np.uint16)
+        hd, app, it = _half_decode(torch_gpu, code, llr_h, 12, flags=nr.EARLY_EXIT)
+        ohd, oapp, oit = oracle.decode_half(oc, llr_h, 12, early_exit=True)
+        assert np.array_equal(it, oit), ("half", kind, Z, m, it, oit)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), ("half", kind, Z, m)
+        stopped["half"] += int(it.min() < 12)
+        q = _quantize(llr)
+        hd, app, it = _char_decode(torch_gpu, code, q, 12, flags=nr.EARLY_EXIT)
+        ohd, oapp, oit, _ = oracle.decode_char(oc, q, 12, early_exit=True)
+        assert np.array_equal(it, oit), ("char", kind, Z, m, it, oit)
+        assert np.array_equal(hd, ohd) and np.array_equal(app, oapp), ("char", kind, Z, m)
+        stopped["char"] += int(it.min() < 12)
+        code.close()
+    # the Eb/N0 lets codewords stop early at most pairs, so the early exit itself is exercised
+    assert min(stopped.values()) > len(PAIRS) // 2, stopped
 
 
 def test_learned_every_lifting_size(nr, oracle, torch_gpu):
