@@ -167,6 +167,8 @@ int decoder_launch_half(int kind, const DecodeArgs& a, hipStream_t s);
 int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s);
 size_t decoder_pair_lds_bytes(int kind, int Z);
 int decoder_pair_slots(int kind, int Z);
+// row groups of the pair kernel for cpb pairs per workgroup (threads = groups * cpb * Z)
+int decoder_pair_groups(int kind, int Z, int cpb);
 // Learned forward that also writes the training records (generic-Z kernel).
 int decoder_launch_train(int kind, const DecodeArgs& a, hipStream_t s);
 

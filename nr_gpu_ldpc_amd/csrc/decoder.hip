@@ -22,6 +22,8 @@ template <int KIND, int ZC, bool LEARNED, bool HALF>
 int launch_decode_slim(const DecodeArgs& a, int grid, int threads, size_t lds, hipStream_t s);
 template <int KIND, int ZC>
 int launch_decode_char_pair(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
+template <int KIND, int ZC>
+int launch_decode_char_split(const DecodeArgs& a, int blocks, int threads, size_t lds, hipStream_t s);
 
 // Lifting sizes with a dedicated kernel (all offsets compile-time): the hot configurations.
 constexpr int kBG1Special = 384, kBG2Special = 208, kQCSpecial = 96;
@@ -207,6 +209,21 @@ int decoder_pair_slots(int kind, int Z) {
     return best;
 }
 
+#ifndef NRLDPC_CHAR_MAX_GROUPS
+#define NRLDPC_CHAR_MAX_GROUPS 2
+#endif
+// Row groups of the CHAR_PRC pair kernel: two (the block rows dealt to two sets of wavefronts,
+// twice the threads) for the base graphs where cpb pairs fill whole wavefronts and twice that
+// fits a workgroup, else one.  Measured at T = 10 on gfx950: two groups decode 1.08-1.49x faster
+// at 1, 2, 4, 5 and 6 wavefronts per group (BG1 Z = 384: 2.75 -> 2.01 ms per 4096 codewords),
+// but slower with three (0.86x at BG1 Z = 192, 0.89x at BG1 Z = 96 with two pairs, 1.03x at
+// BG2 Z = 192), so groups of three wavefronts stay single.
+int decoder_pair_groups(int kind, int Z, int cpb) {
+    const int th = cpb * Z;
+    if (NRLDPC_CHAR_MAX_GROUPS < 2 || (kind != kBG1 && kind != kBG2)) return 1;
+    return th % 64 == 0 && th != 192 && 2 * th <= kSlimMaxBlock ? 2 : 1;
+}
+
 // The CHAR_PRC schedule, two codewords per lane: a.cpb codeword pairs per workgroup, a.lds_cw_bytes
 // LDS bytes per pair.
 int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s) {
@@ -217,6 +234,12 @@ int decoder_launch_char(int kind, const DecodeArgs& a, hipStream_t s) {
     if (threads > kMaxBlock || lds > (size_t)max_lds_bytes() || blocks < 1 || !a.llr_i8) {
         set_last_error("launch shape exceeds the workgroup size or LDS limit");
         return LDPC_EINVAL;
+    }
+    if (decoder_pair_groups(kind, a.Z, a.cpb) == 2) {
+        if (kind == kBG1)
+            return a.Z == kBG1Special ? launch_decode_char_split<kBG1, kBG1Special>(a, blocks, 2 * threads, lds, s)
+                                      : launch_decode_char_split<kBG1, 0>(a, blocks, 2 * threads, lds, s);
+        return launch_decode_char_split<kBG2, 0>(a, blocks, 2 * threads, lds, s);
     }
     if (kind == kBG1)
         return a.Z == kBG1Special ? launch_decode_char_pair<kBG1, kBG1Special>(a, blocks, threads, lds, s)

@@ -10,7 +10,9 @@
 // (v_pk_* instructions): every instruction advances two codewords.
 //
 // One workgroup holds `cpb` codeword pairs; lane (pair slot p, r) owns expanded row r of every
-// block row and sub-column r of every block column of both codewords of pair p.
+// block row and sub-column r of every block column of both codewords of pair p.  With two row
+// groups (k_decode_char_split) the workgroup holds two such lane sets, each running half of the
+// block rows (PairRegs deals them by edge count) and holding only those rows' registers.
 //
 //   VGPRs, per lane   R of every edge into a column of degree > 1, for both codewords, as signed
 //                     bytes (two edges per register: bytes 0/2 = even edge of codeword 2g / 2g+1,
@@ -129,23 +131,46 @@ constexpr int edge_vm_p(int e) {
 
 using CEdge = const EdgeParam __attribute__((address_space(4)));
 
-// Register word of each edge into a column of degree > 1: edges numbered in row-major order
-// skipping the degree-1 columns' edges, two per register.
-template <int KIND>
+// Row groups and register words.  With NG row groups the block rows are dealt to NG sets of
+// wavefronts (each set covers every lane position r of every pair), greedily by edge count, so
+// that each set runs about 1/NG of the check-node work and holds only its own rows' state.
+// Register word of each edge into a column of degree > 1: the group's edges numbered in
+// row-major order skipping the degree-1 columns' edges, two per register; degree-1 columns get
+// register slots among their row's group.
+template <int KIND, int NG = 1>
 struct PairRegs {
     using Sx = Structure<KIND>;
     struct Tab {
-        int idx[Sx::NNZ > 0 ? Sx::NNZ : 1];  // index among the edges into degree > 1 columns, or -1
-        int n;
+        int grp[Sx::ROWS];                   // row group of each block row
+        int idx[Sx::NNZ > 0 ? Sx::NNZ : 1];  // index among the group's edges into degree > 1 columns, or -1
+        int d1[Sx::COLS];                    // register slot of a degree-1 column in its group, else -1
+        int n[NG];                           // edges into degree > 1 columns, per group
+        int nd1[NG];                         // degree-1 columns, per group
     };
     static constexpr Tab make() {
         Tab t{};
-        t.n = 0;
-        for (int e = 0; e < Sx::NNZ; e++) t.idx[e] = Sx::T.nd_index[Sx::T.ecol[e]] >= 0 ? t.n++ : -1;
+        int load[NG] = {};
+        for (int g = 0; g < NG; g++) t.n[g] = t.nd1[g] = 0;
+        for (int i = 0; i < Sx::ROWS; i++) {
+            int g = 0;
+            for (int h = 1; h < NG; h++)
+                if (load[h] < load[g]) g = h;
+            t.grp[i] = g;
+            load[g] += Sx::T.row_start[i + 1] - Sx::T.row_start[i];
+            for (int e = Sx::T.row_start[i]; e < Sx::T.row_start[i + 1]; e++)
+                t.idx[e] = Sx::T.nd_index[Sx::T.ecol[e]] >= 0 ? t.n[g]++ : -1;
+        }
+        for (int j = 0; j < Sx::COLS; j++) t.d1[j] = Sx::T.d1_slot[j] >= 0 ? t.nd1[t.grp[Sx::T.d1_row[j]]]++ : -1;
         return t;
     }
     static constexpr Tab T = make();
-    static constexpr int NRW = (T.n + 1) / 2;
+    static constexpr int max_of(const int* v) {
+        int m = 0;
+        for (int g = 0; g < NG; g++) m = v[g] > m ? v[g] : m;
+        return m;
+    }
+    static constexpr int NRW = (max_of(T.n) + 1) / 2;
+    static constexpr int ND1 = max_of(T.nd1);
     // Whether the other half of edge e's register word belongs to the neighbouring edge of the
     // same row (k: e's position in a row of degree d): then both halves are written together.
     static constexpr bool partner_in_row(int e, int k, int d) {
@@ -166,19 +191,27 @@ __device__ __forceinline__ u32 ld_at(u32 lo, u32 hi, u32 off) {
     return off < 0x10000u ? ld(lo + off) : ld(hi + (off - 0x10000u));
 }
 
-template <int KIND, int ZC, bool EARLY>
-__global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
+// The decoder body.  NG row groups: the workgroup holds NG * cpb * Z lanes, group g being lanes
+// [g cpb Z, (g + 1) cpb Z), whole wavefronts each (the launch guarantees it); a group runs the
+// check nodes of its rows for every pair of the workgroup, and the groups meet only in the LDS
+// accumulator (integer sums, order-free) and at the per-iteration barrier.
+template <int KIND, int ZC, bool EARLY, int NG>
+__device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
     using Sx = Structure<KIND>;
     constexpr int ROWS = Sx::ROWS;
     constexpr int NND = Sx::NND;
-    constexpr int ND1 = Sx::ND1 > 0 ? Sx::ND1 : 1;
-    using PR = PairRegs<KIND>;
+    using PR = PairRegs<KIND, NG>;
+    constexpr int ND1 = PR::ND1 > 0 ? PR::ND1 : 1;
     constexpr int NRW = PR::NRW > 0 ? PR::NRW : 1;
 
     const int Z = ZC > 0 ? ZC : a.Z;
     const int tid = threadIdx.x;
-    const int slot = tid / Z;
-    const int r = tid - slot * Z;
+    const int gsz = a.cpb * Z;  // lanes of one row group
+    // row group of this wavefront (uniform: groups are whole wavefronts)
+    const int grp = NG == 1 ? 0 : __builtin_amdgcn_readfirstlane(tid / gsz);
+    const int tg = tid - grp * gsz;
+    const int slot = tg / Z;
+    const int r = tg - slot * Z;
     const int n_pairs = (a.n_cw + 1) >> 1;
     const int g = blockIdx.x * a.cpb + slot;
     const bool slot_live = g < n_pairs;
@@ -212,7 +245,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
     const int8_t* __restrict__ L1 = a.llr_i8 + (size_t)c1 * a.n_vn;
     auto pair_at = [&](int n) -> s2 { return (s2){(short)L0[n], (short)L1[n]}; };
 #pragma unroll 2
-    for (int j = 0; j < NND; j++) {
+    for (int j = grp; j < NND; j += NG) {
         const s2 x = pair_at(j * Z + r);
         const u32 w = rd_word(x);
         st(rRd + j * 2 * Z4, w);
@@ -224,14 +257,15 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
         constexpr int j = JC;
         constexpr int e = Sx::T.d1_edge[j];
         constexpr int i = Sx::T.d1_row[j];
+        if (PR::T.grp[i] != grp) return;
         if (i < m_rows) {
             int c;
             if constexpr (ZC > 0) c = r + edge_vm_p<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            d1w[Sx::T.d1_slot[j]] = U(pair_at(j * Z + c) << 8);  // R = 0
+            d1w[PR::T.d1[j]] = U(pair_at(j * Z + c) << 8);  // R = 0
         } else {
-            d1w[Sx::T.d1_slot[j]] = 0u;
+            d1w[PR::T.d1[j]] = 0u;
         }
     });
     __syncthreads();
@@ -245,6 +279,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
             constexpr int i = IC;
             constexpr int D = Sx::deg(i);
             constexpr int E0 = Sx::T.row_start[i];
+            if constexpr (NG > 1)
+                if (PR::T.grp[i] != grp) return;
             int mr = m_rows;
             asm volatile("" : "+s"(mr));
             if (i >= mr) return;
@@ -268,7 +304,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                     const u32 off = off2<KIND, ZC, e, j>(et);
                     qv = q_of(ld_at(rRd, rRdHi, off), ro);
                 } else {
-                    const u32 w = d1w[Sx::T.d1_slot[j]];
+                    const u32 w = d1w[PR::T.d1[j]];
                     const s2 x = S(w) >> 8;
                     rod1 = (S(w) << 8) >> 8;
                     qv = rod1 * nf_of(x) + x;
@@ -343,7 +379,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                         else lds_add(rAcHi + (off - 0x10000u), lv);
                     } else {
                         // the degree-1 column's only update: clamped between iterations
-                        u32& w = d1w[Sx::T.d1_slot[j]];
+                        u32& w = d1w[PR::T.d1[j]];
                         const s2 an = (S(w) >> 8) + dt[KC];
                         if (last) {
                             w = U(an);
@@ -362,7 +398,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
         }
         // fold the two halves of every sub-column into the APP, clamp, rebuild the read copy
 #pragma unroll 2
-        for (int j = 0; j < NND; j++) {
+        for (int j = grp; j < NND; j += NG) {
             const u32 sum = ld(rAc + j * 2 * Z4) + ld(rAc + j * 2 * Z4 + Z4);
             const s2 x = smin(smax(from_lin(sum), sp(-128)), sp(127));
             const u32 w = rd_word(x);
@@ -380,7 +416,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
             // every live codeword among its lanes has an unsatisfied check.
             const int lane = tid & 63;
             const int wb = tid - lane;
-            const int lo = max(slot * Z - wb, 0), hi = min((slot + 1) * Z - wb, 64);
+            const int pb = grp * gsz + slot * Z;  // first lane of this lane's pair in its group
+            const int lo = max(pb - wb, 0), hi = min(pb + Z - wb, 64);
             // lanes of this wavefront that belong to this lane's pair (contiguous)
             const uint64_t segm = (hi >= 64 ? ~0ull : ((1ull << hi) - 1ull)) & ~((1ull << lo) - 1ull);
             u32 bad = 0u;
@@ -389,6 +426,8 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                 constexpr int i = IC;
                 constexpr int D = Sx::deg(i);
                 constexpr int E0 = Sx::T.row_start[i];
+                if constexpr (NG > 1)
+                    if (PR::T.grp[i] != grp) return;
                 int mr = m_rows;
                 asm volatile("" : "+s"(mr));
                 if (i >= mr || stop) return;
@@ -399,7 +438,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
                     constexpr int j = Sx::T.ecol[e];
                     s2 x;
                     if constexpr (j < NND) x = S(ld_at(rRd, rRdHi, off2<KIND, ZC, e, j>(et))) >> 8;
-                    else x = S(d1w[Sx::T.d1_slot[j]]) >> 8;
+                    else x = S(d1w[PR::T.d1[j]]) >> 8;
                     p ^= U(x - sp(1));
                 });
                 bad |= p;
@@ -441,7 +480,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
     float* ap0 = a.app_out ? a.app_out + (size_t)c0 * nv : nullptr;
     float* ap1 = a.app_out ? a.app_out + (size_t)c1 * nv : nullptr;
     const bool w0 = slot_live && ap0 != nullptr, w1 = hi_real && ap1 != nullptr;
-    for (int j = 0; j < NND; j++) {
+    for (int j = grp; j < NND; j += NG) {
         const s2 x = from_lin(ld(rAc + j * 2 * Z4) + ld(rAc + j * 2 * Z4 + Z4));
         const int col = j * Z + r;
         stb(STG + col, x.x > 0 ? 0 : 1);
@@ -453,12 +492,12 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
         constexpr int j = JC;
         constexpr int e = Sx::T.d1_edge[j];
         constexpr int i = Sx::T.d1_row[j];
-        if (i < m_rows) {
+        if (i < m_rows && PR::T.grp[i] == grp) {
             int c;
             if constexpr (ZC > 0) c = r + edge_vm_p<KIND, ZC>(e);
             else c = r + (int)et[e].off_rd;
             if (c >= Z) c -= Z;
-            const u32 w = d1w[Sx::T.d1_slot[j]];
+            const u32 w = d1w[PR::T.d1[j]];
             const s2 x = ran_last ? S(w) : S(w) >> 8;
             const int col = j * Z + c;
             stb(STG + col, x.x > 0 ? 0 : 1);
@@ -469,7 +508,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
     });
     // zero padding up to the next multiple of 8 columns (stepping by Z: fewer than 8 lanes per
     // codeword pair when Z < 8)
-    for (u32 p = (u32)r; (u32)nv + p < nvp; p += (u32)Z) {
+    for (u32 p = (u32)(r + grp * Z); (u32)nv + p < nvp; p += (u32)(NG * Z)) {
         stb(STG + nv + p, 0);
         stb(STG + nvp + nv + p, 0);
     }
@@ -479,17 +518,30 @@ __global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
         if (!(h == 0 ? slot_live : hi_real)) continue;
         uint8_t* hrow = a.hd + (size_t)(h == 0 ? c0 : c1) * hb;
         const u32 src = STG + (h == 0 ? 0u : nvp);
-        for (int b = r; b < hb; b += Z) {
+        for (int b = r + grp * Z; b < hb; b += NG * Z) {
             const u32 v0 = ld(src + 8u * b), v1 = ld(src + 8u * b + 4u);
             const u32 lo4 = (v0 | (v0 >> 7) | (v0 >> 14) | (v0 >> 21)) & 0xFu;
             const u32 hi4 = (v1 | (v1 >> 7) | (v1 >> 14) | (v1 >> 21)) & 0xFu;
             hrow[b] = (uint8_t)(lo4 | (hi4 << 4));
         }
     }
-    if (r == 0 && a.iters_out) {
+    if (r == 0 && grp == 0 && a.iters_out) {
         if (slot_live) a.iters_out[c0] = (int)(its & 0xFFFFu);
         if (hi_real) a.iters_out[c1] = (int)(its >> 16);
     }
+}
+
+template <int KIND, int ZC, bool EARLY>
+__global__ void __launch_bounds__(kMaxBlock) k_decode_char_pair(DecodeArgs a) {
+    decode_pair<KIND, ZC, EARLY, 1>(a);
+}
+
+// Two row groups (2 cpb Z lanes, whole wavefronts per group): at BG1 Z = 384 twelve wavefronts
+// per workgroup, three on each SIMD, in place of six placed 2-2-1-1.
+template <int KIND, int ZC, bool EARLY>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, kSlimMaxBlock), amdgpu_waves_per_eu(3)))
+k_decode_char_split(DecodeArgs a) {
+    decode_pair<KIND, ZC, EARLY, 2>(a);
 }
 
 }  // namespace pairk
