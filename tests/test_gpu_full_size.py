@@ -45,6 +45,39 @@ def test_bg1_z384_batch_4096_samples_match_oracle(nr, oracle, torch_gpu):
     assert (its.cpu().numpy() == 10).all()
 
 
+@pytest.mark.parametrize("ebn0,early", [(1.5, False), (2.25, True)])
+def test_bg1_z384_char_batch_4096_samples_match_oracle(nr, oracle, torch_gpu, ebn0, early):
+    """The CHAR_PRC launch of the same workload (the reference's default build: LLRs as
+    quantize(0.5, x, 8), 2048 workgroups of one codeword pair in two row groups): sampled
+    codewords from both halves of pairs and both ends of the batch equal the oracle's CHAR_PRC
+    decode bit for bit, with early exit too (iteration counts included)."""
+    torch = torch_gpu
+    print(f"case CHAR BG1 Z=384 batch 4096 T=10 ebn0={ebn0} early={early}", flush=True)
+    code = nr.Code(1, 384)
+    n = 4096
+    llr = _channel(nr, torch, code, n, ebn0, seed=405)
+    q = torch.clamp(torch.floor(llr / 0.5), -128, 127).to(torch.int8)  # quantize(0.5, x, 8)
+    del llr
+    hd = torch.zeros((n, code.hd_bytes), dtype=torch.uint8, device="cuda")
+    app = torch.zeros((n, code.n_vn), dtype=torch.float32, device="cuda")
+    its = torch.zeros(n, dtype=torch.int32, device="cuda")
+    code.decode_char(q.data_ptr(), n, 10, hd.data_ptr(), app.data_ptr(), its.data_ptr(),
+                     flags=nr.EARLY_EXIT if early else 0, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    pick = np.array([0, 1, 776, 777, 2046, 2047, 3001, 4094, 4095])
+    sel = torch.from_numpy(pick).cuda()
+    ohd, oapp, oit, _ = oracle.decode_char(oracle.code(OR_BG1, 384), q[sel].cpu().numpy(), 10, early_exit=early)
+    ghd = np.unpackbits(hd[sel].cpu().numpy(), axis=1, bitorder="little")[:, :code.n_vn]
+    assert np.array_equal(ghd, ohd)
+    assert np.array_equal(app[sel].cpu().numpy(), oapp)
+    git = its.cpu().numpy()
+    assert np.array_equal(git[pick], oit)
+    if early:
+        assert git.min() < 10 and git.max() == 10  # codewords stop at different iterations
+    else:
+        assert (git == 10).all()
+
+
 def test_more_than_2_pow_31_llr_elements(nr, oracle, torch_gpu):
     """82,300 BG1 Z=384 codewords hold 2.149e9 LLRs (8.6 GB): offsets past 2^31 elements are
     addressed correctly; sampled codewords from the far end of the batch equal the oracle."""
