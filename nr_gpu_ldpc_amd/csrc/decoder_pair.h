@@ -70,6 +70,14 @@ typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s2 usub_sat(s2 a, s2 b) {
     return __builtin_bit_cast(s2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, a), __builtin_bit_cast(u2, b)));
 }
+// a - b per half, saturated to [-32768, 32767]: one v_pk_sub_i16 with clamp
+__device__ __forceinline__ s2 ssub_sat(s2 a, s2 b) { return __builtin_elementwise_sub_sat(a, b); }
+// x * 256 per half as unsigned 16-bit (x in [0, 255])
+__device__ __forceinline__ s2 shl8(s2 x) { return __builtin_bit_cast(s2, __builtin_bit_cast(u2, x) << (unsigned short)8); }
+// min per half, unsigned
+__device__ __forceinline__ s2 umin(s2 a, s2 b) {
+    return __builtin_bit_cast(s2, __builtin_elementwise_min(__builtin_bit_cast(u2, a), __builtin_bit_cast(u2, b)));
+}
 
 extern __shared__ __attribute__((aligned(16))) unsigned char g_psmem[];
 using lds_u32 = __attribute__((address_space(3))) uint32_t;
@@ -115,6 +123,12 @@ template <int N>
 __device__ __forceinline__ s2 r_get(const u32* RW) {
     if constexpr (N & 1) return S(RW[N >> 1]) >> 8;
     else return (S(RW[N >> 1]) << 8) >> 8;
+}
+// The same R times 256 (its byte in the high byte of each half, low byte 0).
+template <int N>
+__device__ __forceinline__ s2 r256_get(const u32* RW) {
+    if constexpr (N & 1) return S(RW[N >> 1] & 0xFF00FF00u);
+    else return S(RW[N >> 1]) << 8;
 }
 
 template <int KIND, int ZC>
@@ -288,8 +302,8 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
             // rows of at most 12 edges hold |Q| and R_old from pass 1 to pass 2; wider ones (the
             // register peak) recompute them there
             constexpr bool KEEP_A = D <= 12;
-            s2 q[D], aq[KEEP_A ? D : 1], rok[KEEP_A ? D : 1];
-            s2 rod1 = sp(0);  // R_old of the row's degree-1 edge, if any
+            s2 q[D], aq[KEEP_A ? D : 1];
+            s2 rod1 = sp(0), rod1_256 = sp(0);  // R_old of the row's degree-1 edge (and times 256), if any
             s2 m1 = sp(1000), m2 = sp(1000);
             u32 par = 0u;
             // pass 1: Q, |Q|, the two minima, the sign parity
@@ -300,13 +314,13 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
                 s2 qv;
                 if constexpr (j < NND) {
                     const s2 ro = r_get<PR::T.idx[e]>(RW);
-                    if constexpr (KEEP_A) rok[k] = ro;
                     const u32 off = off2<KIND, ZC, e, j>(et);
                     qv = q_of(ld_at(rRd, rRdHi, off), ro);
                 } else {
                     const u32 w = d1w[PR::T.d1[j]];
                     const s2 x = S(w) >> 8;
-                    rod1 = (S(w) << 8) >> 8;
+                    rod1_256 = S(w) << 8;
+                    rod1 = rod1_256 >> 8;
                     qv = rod1 * nf_of(x) + x;
                 }
                 const s2 av = smax(qv, sp(0) - qv);
@@ -316,10 +330,13 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
                 m1 = smin(m1, av);
                 par ^= U(qv);
             });
-            // scaled magnitudes with the product sign: +-min(trunc(0.75 m), 128)
+            // scaled magnitudes with the product sign, times 256: +-min(256 trunc(0.75 m), 32767).
+            // Pass 2 keeps R times 256 (R in the high byte): 32767 and -32767 stand for R = 127
+            // and -128 (the clamp of +128 to 127 comes for free), and dt = clamp(R - R_old, -128,
+            // 127) is the high byte of the saturating difference of the two scaled values.
             const s2 pm = S(par) >> 15;
-            const s2 n1 = smin((m1 * sp(3)) >> 2, sp(128));
-            const s2 n2 = smin((m2 * sp(3)) >> 2, sp(128));
+            const s2 n1 = umin(shl8(m1 * sp(3) >> 2), sp(32767));
+            const s2 n2 = umin(shl8(m2 * sp(3) >> 2), sp(32767));
             const s2 n1s = (n1 ^ pm) - pm, n2s = (n2 ^ pm) - pm;
             const s2 d21 = n2s - n1s;
             const s2 m1p = m1 + sp(1);
@@ -342,19 +359,16 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
                 });
                 sfor<0, CN>([&](auto KC) { v[KC] = v[KC] * d21 + n1s; });  // +-scaled magnitude
                 sfor<0, CN>([&](auto KC) { v[KC] = v[KC] ^ sq[KC]; });
-                sfor<0, CN>([&](auto KC) { v[KC] = v[KC] - sq[KC]; });    // own sign
-                sfor<0, CN>([&](auto KC) { v[KC] = smin(v[KC], sp(127)); });  // R_new
+                sfor<0, CN>([&](auto KC) { v[KC] = v[KC] - sq[KC]; });    // own sign: R_new times 256
                 s2 dt[CN];
                 sfor<0, CN>([&](auto KC) {
                     constexpr int e = E0 + K0 + KC;
                     constexpr int j = Sx::T.ecol[e];
-                    if constexpr (j >= NND) dt[KC] = v[KC] - rod1;
-                    else if constexpr (KEEP_A) dt[KC] = v[KC] - rok[K0 + KC];
-                    else dt[KC] = v[KC] - r_get<PR::T.idx[e]>(RW);
+                    if constexpr (j >= NND) dt[KC] = ssub_sat(v[KC], rod1_256);
+                    else dt[KC] = ssub_sat(v[KC], r256_get<PR::T.idx[e]>(RW));
                 });
-                sfor<0, CN>([&](auto KC) { dt[KC] = smax(dt[KC], sp(-128)); });
                 sfor<0, CN>([&](auto KC) {
-                    dt[KC] = smin(dt[KC], sp(127));
+                    dt[KC] = dt[KC] >> 8;
                     if constexpr (EARLY) dt[KC] = S(U(dt[KC]) & live);
                 });
                 sfor<0, CN>([&](auto KC) {
@@ -366,12 +380,13 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
                         // store R (R_old of this edge has been read)
                         constexpr int n = PR::T.idx[e];
                         constexpr bool partner_here = PR::partner_in_row(e, k, D);
+                        // rn holds R in the high byte of each half
                         if constexpr (n & 1) {
-                            if constexpr (partner_here) RW[n >> 1] = __builtin_amdgcn_perm(U(rn), U(pend), 0x06020400u);
-                            else RW[n >> 1] = __builtin_amdgcn_perm(U(rn), RW[n >> 1], 0x06020400u);
+                            if constexpr (partner_here) RW[n >> 1] = __builtin_amdgcn_perm(U(rn), U(pend), 0x07030501u);
+                            else RW[n >> 1] = __builtin_amdgcn_perm(U(rn), RW[n >> 1], 0x07020500u);
                         } else {
                             if constexpr (partner_here) pend = rn;
-                            else RW[n >> 1] = (RW[n >> 1] & 0xFF00FF00u) | (U(rn) & 0x00FF00FFu);
+                            else RW[n >> 1] = __builtin_amdgcn_perm(RW[n >> 1], U(rn), 0x07030501u);
                         }
                         const u32 off = off2<KIND, ZC, e, j>(et);
                         const u32 lv = to_lin(dt[KC], k8000, kneg);
@@ -385,7 +400,7 @@ __device__ __forceinline__ void decode_pair(const DecodeArgs& a) {
                             w = U(an);
                         } else {
                             const s2 ac = smin(smax(an, sp(-128)), sp(127));
-                            w = U(ac << 8) | (U(rn) & 0x00FF00FFu);
+                            w = __builtin_amdgcn_perm(U(ac << 8), U(rn), 0x07030501u);  // [R, A, R, A]
                         }
                     }
                 });
